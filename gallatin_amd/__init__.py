@@ -1,0 +1,316 @@
+"""Python host interface of the MI355X Gallatin allocator (libgallatin_hip.so).
+
+Mirrors the reference's host API (include/gallatin/allocators/gallatin.cuh
+and global_allocator.cuh of saltsystemslab/gallatin) over the C ABI declared
+in include/gallatin_capi.h:
+
+    alloc = Gallatin(16 << 20, 16, 4096).generate_on_device(num_bytes, seed)
+    alloc.print_info();  alloc.free_on_device()
+    init_global_allocator(num_bytes, seed); print_global_stats(); free_global_allocator()
+
+plus the workload kernels the reference's tests run (bulk malloc/free, the
+fixed-size and churn tests) and a single-wave deterministic stream.  Device
+buffers are torch tensors on ``cuda``; torch is only the memory/stream
+plumbing.  Everything fails loudly: a missing library raises ImportError on
+first use, a failing call raises GallatinError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgallatin_hip.so")
+FAIL = (1 << 64) - 1
+
+GAL_FLAG_DETERMINISTIC = 1
+GAL_FLAG_NO_PAYLOAD_FENCE = 2
+
+CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
+CTR_COUNT = 8
+
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+_p = ctypes.c_void_p
+
+# symbols declared in include/gallatin_capi.h
+CAPI_SYMBOLS = [
+    "gal_config_default", "gal_create", "gal_destroy", "gal_device_state",
+    "gal_from_device_state", "gal_print_info", "gal_get_info", "gal_drain",
+    "gal_reboot", "gal_snapshot", "gal_tree_words", "gal_init_global",
+    "gal_free_global", "gal_print_global_stats", "gal_global_handle",
+    "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
+    "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
+    "gal_run_churn", "gal_bench_atomic", "gal_bench_copy", "gal_last_error",
+]
+
+
+class GallatinError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("max_bytes", _u64), ("bytes_per_segment", _u64),
+                ("smallest", _u64), ("biggest", _u64), ("seed", _u64),
+                ("flags", _u32), ("slots0", _u32), ("slots_min", _u32),
+                ("print_info", _u32), ("request_attempts", _u32),
+                ("slice_attempts", _u32), ("block_attempts", _u32),
+                ("segment_attempts", _u32)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("num_segments", _u64), ("bytes_per_segment", _u64),
+                ("smallest", _u64), ("biggest", _u64),
+                ("blocks_per_segment", _u64), ("num_trees", _u32),
+                ("total_slots", _u32), ("segments_free", _u64),
+                ("segments_formatted", _u64), ("segments_multi", _u64),
+                ("tree_owned", _u64 * 24), ("errors", _u64),
+                ("overhead_bytes", _u64), ("pool_base", _u64)]
+
+
+STEP_DTYPE = np.dtype([("kind", "<u4"), ("pad", "<u4"), ("mask", "<u8"),
+                       ("arg", "<u8", (64,))])
+
+_lib = None
+
+
+def lib():
+    """Load libgallatin_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m gallatin_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.gal_config_default.argtypes = [ctypes.POINTER(Config), _u64, _u64]
+        L.gal_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_p)]
+        L.gal_destroy.argtypes = [_p]
+        L.gal_device_state.restype = _p
+        L.gal_device_state.argtypes = [_p]
+        L.gal_from_device_state.restype = _p
+        L.gal_from_device_state.argtypes = [_p]
+        L.gal_print_info.argtypes = [_p]
+        L.gal_get_info.argtypes = [_p, ctypes.POINTER(Info)]
+        L.gal_drain.argtypes = [_p, _p]
+        L.gal_reboot.argtypes = [_p, _p]
+        L.gal_snapshot.argtypes = [_p, _p, _p, _p]
+        L.gal_tree_words.restype = ctypes.c_int64
+        L.gal_tree_words.argtypes = [_p, ctypes.c_int, _p, _u64, _p, _p]
+        L.gal_init_global.argtypes = [_u64, _u64, ctypes.c_int]
+        L.gal_global_handle.restype = _p
+        L.gal_malloc_bulk.argtypes = [_p, _p, _u64, _p, _u32, _p]
+        L.gal_free_bulk.argtypes = [_p, _p, _u64, _u32, _p]
+        L.gal_global_malloc_bulk.argtypes = [_p, _u64, _p, _u32, _p]
+        L.gal_global_free_bulk.argtypes = [_p, _u64, _u32, _p]
+        L.gal_run_wave_stream.argtypes = [_p, _p, _u64, _p, _p]
+        L.gal_run_plain.argtypes = [_p, _u64, _u64, _p, _u32, _p]
+        L.gal_run_churn.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _p]
+        L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
+        L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
+        L.gal_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise GallatinError(lib().gal_last_error().decode())
+    return rc
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev_u64(n):
+    import torch
+    return torch.zeros(max(int(n), 1), dtype=torch.int64, device="cuda")
+
+
+class Gallatin:
+    """Gallatin<bytes_per_segment, smallest, biggest> (gallatin.cuh:292)."""
+
+    def __init__(self, bytes_per_segment=16 << 20, smallest=16, biggest=4096):
+        self.bytes_per_segment = bytes_per_segment
+        self.smallest = smallest
+        self.biggest = biggest
+        self.handle = None
+
+    # ---- host lifecycle (gallatin.cuh:324-421, 643-668, 1855) ----
+    def generate_on_device(self, max_bytes, seed, print_info=True,
+                           running_calloc=False, *, flags=0, slots0=128,
+                           slots_min=4, limits=None):
+        """limits: (request, slice, block, segment) retry limits; None or 0
+        entries keep the reference's 1000, 5000, 500, 500."""
+        if running_calloc:
+            raise GallatinError("calloc mode is not supported")
+        cfg = Config()
+        lib().gal_config_default(ctypes.byref(cfg), max_bytes, seed)
+        cfg.bytes_per_segment = self.bytes_per_segment
+        cfg.smallest = self.smallest
+        cfg.biggest = self.biggest
+        cfg.flags = flags
+        cfg.slots0 = slots0
+        cfg.slots_min = slots_min
+        cfg.print_info = int(bool(print_info))
+        if limits:
+            (cfg.request_attempts, cfg.slice_attempts, cfg.block_attempts,
+             cfg.segment_attempts) = [int(x or 0) for x in limits]
+        h = _p()
+        _check(lib().gal_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self.flags = flags
+        self._info_static()
+        return self
+
+    def _info_static(self):
+        i = self.info()
+        self.num_segments = i["num_segments"]
+        self.num_trees = i["num_trees"]
+        self.blocks_per_segment = i["blocks_per_segment"]
+        self.total_slots = i["total_slots"]
+        self.pool_base = i["pool_base"]
+
+    def free_on_device(self):
+        if self.handle:
+            _check(lib().gal_destroy(self.handle))
+            self.handle = None
+
+    close = free_on_device
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free_on_device()
+
+    def device_state(self):
+        return lib().gal_device_state(self.handle)
+
+    def print_info(self):
+        _check(lib().gal_print_info(self.handle))
+
+    def info(self):
+        i = Info()
+        _check(lib().gal_get_info(self.handle, ctypes.byref(i)))
+        d = {f: getattr(i, f) for f, _ in Info._fields_ if f != "tree_owned"}
+        d["tree_owned"] = list(i.tree_owned)[: i.num_trees]
+        return d
+
+    def drain(self):
+        _check(lib().gal_drain(self.handle, _stream()))
+        import torch
+        torch.cuda.synchronize()
+
+    def reboot(self):
+        _check(lib().gal_reboot(self.handle, _stream()))
+
+    def snapshot(self):
+        """(seg_state u64[N], slots u32[total_slots], blocks u32[N*B, 2])."""
+        n = self.num_segments
+        st = np.zeros(n, np.uint64)
+        sl = np.zeros(self.total_slots, np.uint32)
+        bl = np.zeros(2 * n * self.blocks_per_segment, np.uint32)
+        _check(lib().gal_snapshot(self.handle, st.ctypes.data, sl.ctypes.data,
+                                  bl.ctypes.data))
+        return st, sl, bl.reshape(-1, 2)
+
+    def tree_layers(self, tree):
+        """Words of each layer (top first); tree -1 = segment tree."""
+        nl = _u32()
+        lw = (_u64 * 4)()
+        n = _check(lib().gal_tree_words(self.handle, tree, None, 0,
+                                        ctypes.byref(nl), lw))
+        out = np.zeros(n, np.uint64)
+        _check(lib().gal_tree_words(self.handle, tree, out.ctypes.data, n,
+                                    ctypes.byref(nl), lw))
+        layers, off = [], 0
+        for layer in range(nl.value):
+            layers.append(out[off:off + lw[layer]].copy())
+            off += lw[layer]
+        return layers
+
+    # ---- workload kernels (tests/src/*.cu) ----
+    def malloc_bulk(self, sizes, block_threads=256):
+        """sizes: int64 cuda tensor -> int64 cuda tensor of device pointers."""
+        import torch
+        out = torch.zeros(sizes.numel(), dtype=torch.int64, device="cuda")
+        _check(lib().gal_malloc_bulk(self.handle, sizes.data_ptr(), sizes.numel(),
+                                     out.data_ptr(), block_threads, _stream()))
+        return out
+
+    def free_bulk(self, ptrs, block_threads=256):
+        _check(lib().gal_free_bulk(self.handle, ptrs.data_ptr(), ptrs.numel(),
+                                   block_threads, _stream()))
+
+    def run_wave_stream(self, steps):
+        """Replay wave steps (STEP_DTYPE) on one wave; byte offsets or FAIL."""
+        import torch
+        steps = np.ascontiguousarray(steps, dtype=STEP_DTYPE)
+        raw = torch.from_numpy(steps.view(np.uint8).copy()).cuda()
+        res = torch.zeros(len(steps) * 64, dtype=torch.int64, device="cuda")
+        _check(lib().gal_run_wave_stream(self.handle, raw.data_ptr(), len(steps),
+                                         res.data_ptr(), _stream()))
+        torch.cuda.synchronize()
+        return res.cpu().numpy().view(np.uint64)
+
+    def run_plain(self, nthreads, size, counters=None, block_threads=256):
+        ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+        _check(lib().gal_run_plain(self.handle, nthreads, size, ctr.data_ptr(),
+                                   block_threads, _stream()))
+        return ctr
+
+    def run_churn(self, seed, nthreads, rounds, counters=None, block_threads=256):
+        ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+        _check(lib().gal_run_churn(self.handle, seed, nthreads, rounds,
+                                   ctr.data_ptr(), block_threads, _stream()))
+        return ctr
+
+
+# ---- global allocator (global_allocator.cuh:41-97) ----
+def init_global_allocator(num_bytes, seed, print_info=True, running_calloc=False):
+    if running_calloc:
+        raise GallatinError("calloc mode is not supported")
+    _check(lib().gal_init_global(num_bytes, seed, int(bool(print_info))))
+
+
+def free_global_allocator():
+    _check(lib().gal_free_global())
+
+
+def print_global_stats():
+    _check(lib().gal_print_global_stats())
+
+
+def global_allocator():
+    """Non-owning view of the global instance (for info / snapshots)."""
+    h = lib().gal_global_handle()
+    if not h:
+        raise GallatinError("no global allocator")
+    g = Gallatin()
+    g.handle = h
+    g.flags = 0
+    g._info_static()
+    return g
+
+
+def global_malloc_bulk(sizes, block_threads=256):
+    import torch
+    out = torch.zeros(sizes.numel(), dtype=torch.int64, device="cuda")
+    _check(lib().gal_global_malloc_bulk(sizes.data_ptr(), sizes.numel(),
+                                        out.data_ptr(), block_threads, _stream()))
+    return out
+
+
+def global_free_bulk(ptrs, block_threads=256):
+    _check(lib().gal_global_free_bulk(ptrs.data_ptr(), ptrs.numel(),
+                                      block_threads, _stream()))
+
+
+def bench_atomic(words, span, n_ops, lanes_per_address=1, stride=1):
+    _check(lib().gal_bench_atomic(words.data_ptr(), span, n_ops,
+                                  lanes_per_address, stride, _stream()))
+
+
+def bench_copy(src, dst, nbytes):
+    _check(lib().gal_bench_copy(src.data_ptr(), dst.data_ptr(), nbytes, _stream()))

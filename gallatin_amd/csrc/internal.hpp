@@ -1,0 +1,29 @@
+// Host-side handle of one allocator instance (private to libgallatin_hip).
+#ifndef GALLATIN_CAPI_INTERNAL_HPP
+#define GALLATIN_CAPI_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "gallatin/allocators/layout.hpp"
+#include "gallatin_capi.h"
+
+struct gal_allocator {
+  gallatin::allocators::gal_state host;  // host copy of the device object
+  gallatin::allocators::gal_state* dev;  // device object (inside `meta`)
+  void* meta = nullptr;                  // metadata arena
+  uint64_t meta_bytes = 0;
+  void* pool = nullptr;
+  uint64_t pool_bytes = 0;
+  uint64_t* tree_base = nullptr;         // device address of the tree arena
+  uint64_t tree_words = 0;
+  int device = 0;
+};
+
+namespace gallatin_capi {
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+}  // namespace gallatin_capi
+
+#endif

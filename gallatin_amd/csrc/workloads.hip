@@ -1,0 +1,276 @@
+// Workload kernels behind the C ABI: bulk malloc/free, the single-wave
+// deterministic stream, the plain fixed-size method, the mixed-size churn and
+// the atomic / copy ceilings used for the roofline.
+//
+// Reference counterparts: tests/src/gallatin_test.cu:59-112
+// (alloc_one_size_pointer / free_one_size_pointer) and
+// tests/src/gallatin_churn.cu:37-95 (pointer_churn_kernel).
+#include <hip/hip_runtime.h>
+
+#include "gallatin/allocators/gallatin.hpp"
+#include "gallatin/streams.h"
+#include "gallatin_capi.h"
+#include "internal.hpp"
+
+using namespace gallatin;
+using namespace gallatin::allocators;
+using namespace gallatin_capi;
+
+namespace {
+
+// Sum of a per-lane value over the wave (every lane of the wave active).
+__device__ inline uint64_t wave_sum(uint64_t v) {
+  for (int off = 32; off >= 1; off >>= 1)
+    v += wave::fetch(v, (int)(wave::lane() ^ (uint32_t)off));
+  return v;
+}
+
+__device__ inline void add_counters(uint64_t* ctr, const uint64_t* v, int n) {
+  for (int i = 0; i < n; ++i) {
+    const uint64_t s = wave_sum(v[i]);
+    if (wave::lane() == 0 && s) utils::atomic_add(&ctr[i], (uint64_t)s);
+  }
+}
+
+// The wave writes each participating lane's buffer in turn, 16 bytes per
+// lane per step, so every store instruction is one contiguous 1 KiB run.
+__device__ inline void wave_fill(bool on, uint64_t* p, uint64_t words, uint64_t pat) {
+  const uint64_t act = wave::active();
+  const uint32_t k = wave::count(act);
+  const uint32_t rank = wave::count(act & wave::lanes_below());
+  for (uint64_t m = __ballot(on); m; m &= m - 1) {
+    const int l = wave::first(m);
+    uint64_t* q = reinterpret_cast<uint64_t*>(wave::readlane((uint64_t)p, l));
+    const uint64_t n = wave::readlane(words, l);
+    const uint64_t v = wave::readlane(pat, l);
+    for (uint64_t i = 2ULL * rank; i < n; i += 2ULL * k) {
+      if (i + 1 < n)
+        *reinterpret_cast<ulonglong2*>(q + i) = make_ulonglong2(v, v);
+      else
+        q[i] = v;
+    }
+  }
+}
+
+// Cooperative check of the same layout; returns this lane's mismatch count.
+__device__ inline uint64_t wave_verify(bool on, const uint64_t* p, uint64_t words,
+                                       uint64_t pat) {
+  const uint64_t act = wave::active();
+  const uint32_t k = wave::count(act);
+  const uint32_t rank = wave::count(act & wave::lanes_below());
+  uint64_t bad = 0;
+  for (uint64_t m = __ballot(on); m; m &= m - 1) {
+    const int l = wave::first(m);
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(wave::readlane((uint64_t)p, l));
+    const uint64_t n = wave::readlane(words, l);
+    const uint64_t v = wave::readlane(pat, l);
+    for (uint64_t i = 2ULL * rank; i < n; i += 2ULL * k) {
+      if (i + 1 < n) {
+        const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(q + i);
+        bad += (x.x != v) + (x.y != v);
+      } else {
+        bad += q[i] != v;
+      }
+    }
+  }
+  return bad;
+}
+
+}  // namespace
+
+__global__ void k_malloc_bulk(gal_state* s, const uint64_t* sizes, uint64_t n, void** out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = detail::malloc_impl(s, sizes[i]);
+}
+
+__global__ void k_free_bulk(gal_state* s, void* const* ptrs, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  void* p = ptrs[i];
+  if (p) detail::free_impl(s, p);
+}
+
+// One wave replays a stream of wave steps in order.
+__global__ void k_wave_stream(gal_state* s, const gal_wave_step* steps, uint64_t nsteps,
+                              uint64_t* results) {
+  const uint32_t lane = threadIdx.x;
+  const uint64_t base = (uint64_t)s->memory;
+  for (uint64_t i = 0; i < nsteps; ++i) {
+    const uint32_t kind = steps[i].kind;
+    const uint64_t mask = steps[i].mask;
+    const bool on = (mask >> lane) & 1;
+    uint64_t res = kFail;
+    if (kind == 0) {
+      if (on) {
+        void* p = detail::malloc_impl(s, steps[i].arg[lane]);
+        if (p) res = (uint64_t)p - base;
+      }
+    } else if (on) {
+      const uint64_t ref = steps[i].arg[lane];
+      const uint64_t off = ref < i * 64 ? results[ref] : kFail;  // earlier steps only
+      detail::free_impl(s, off == kFail ? nullptr : (void*)(base + off));
+    }
+    results[i * 64 + lane] = res;
+    __threadfence_block();
+  }
+}
+
+// Plain method: malloc(size), write the thread id, read it back, free.
+__global__ void k_plain(gal_state* s, uint64_t n, uint64_t size, uint64_t* ctr) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t c[5] = {0, 0, 0, 0, 0};
+  if (tid < n) {
+    volatile uint32_t* p = static_cast<volatile uint32_t*>(detail::malloc_impl(s, size));
+    c[GAL_CTR_MALLOCS] = 1;
+    if (!p) {
+      c[GAL_CTR_MISSES] = 1;
+    } else {
+      p[0] = (uint32_t)tid;
+      c[GAL_CTR_BAD] = p[0] != (uint32_t)tid;
+      c[GAL_CTR_BYTES] = 8;
+      detail::free_impl(s, (void*)p);
+      c[GAL_CTR_PAIRS] = 1;
+    }
+  }
+  add_counters(ctr, c, 5);
+}
+
+// Mixed-size churn: one live allocation per thread, replaced in round 0, in
+// every later round whose coin is set, and freed after the last round.
+__global__ void k_churn(gal_state* s, uint64_t seed, uint64_t n, uint32_t rounds,
+                        uint64_t* ctr) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = tid < n;
+  uint64_t* live = nullptr;
+  uint64_t live_words = 0;
+  uint32_t live_round = 0;
+  uint64_t c[5] = {0, 0, 0, 0, 0};
+  for (uint32_t r = 0; r <= rounds; ++r) {
+    const bool replace =
+        active && (r == 0 || r == rounds || gal_churn_coin(seed, tid, r) != 0);
+    const bool do_free = replace && live != nullptr;
+    c[GAL_CTR_BAD] += wave_verify(do_free, live, live_words, gal_churn_pattern(tid, live_round));
+    if (do_free) {
+      detail::free_impl(s, live);
+      c[GAL_CTR_PAIRS] += 1;
+      c[GAL_CTR_BYTES] += live_words * 8;
+      live = nullptr;
+    }
+    if (r == rounds) break;
+    const uint64_t size = gal_churn_size(seed, tid, r);
+    uint64_t* np = nullptr;
+    if (replace) {
+      np = static_cast<uint64_t*>(detail::malloc_impl(s, size));
+      c[GAL_CTR_MALLOCS] += 1;
+      if (!np) c[GAL_CTR_MISSES] += 1;
+    }
+    const bool filled = replace && np != nullptr;
+    if (filled) {
+      live = np;
+      live_words = gal_fill_words(size);
+      live_round = r;
+      c[GAL_CTR_BYTES] += live_words * 8;
+    }
+    wave_fill(filled, np, gal_fill_words(size), gal_churn_pattern(tid, r));
+  }
+  add_counters(ctr, c, 5);
+}
+
+// Atomic-rate probe: op i adds 1 to word ((i / lanes_per_address) * stride)
+// % span and keeps the returned value live, as the allocator's claims do.
+__global__ void k_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,
+                               uint32_t lanes_per_address, uint32_t stride) {
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_ops; i += nthreads) {
+    const uint64_t idx = ((i / lanes_per_address) * stride) % span;
+    acc += utils::atomic_add(&words[idx], 1u);
+  }
+  if (acc == 0xFFFFFFFFu) words[0] = acc;
+}
+
+__global__ void k_bench_copy(const uint4* src, uint4* dst, uint64_t n) {
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nthreads)
+    dst[i] = src[i];
+}
+
+static unsigned grid_for(uint64_t n, uint32_t threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+static int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, what);
+}
+
+extern "C" {
+
+int gal_malloc_bulk(gal_allocator* a, const uint64_t* sizes, uint64_t n, void** out,
+                    uint32_t threads, void* stream) {
+  if (!a || (!sizes && n) || (!out && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
+  k_malloc_bulk<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, sizes, n, out);
+  return launch_status("malloc_bulk");
+}
+
+int gal_free_bulk(gal_allocator* a, void* const* ptrs, uint64_t n, uint32_t threads,
+                  void* stream) {
+  if (!a || (!ptrs && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
+  k_free_bulk<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, ptrs, n);
+  return launch_status("free_bulk");
+}
+
+int gal_run_wave_stream(gal_allocator* a, const gal_wave_step* steps, uint64_t nsteps,
+                        uint64_t* results, void* stream) {
+  if (!a || (!steps && nsteps) || (!results && nsteps)) return fail(GAL_ERR_INVALID, "null argument");
+  if (nsteps == 0) return GAL_OK;
+  k_wave_stream<<<1, 64, 0, (hipStream_t)stream>>>(a->dev, steps, nsteps, results);
+  return launch_status("wave_stream");
+}
+
+int gal_run_plain(gal_allocator* a, uint64_t n, uint64_t size, uint64_t* ctr,
+                  uint32_t threads, void* stream) {
+  if (!a || !ctr) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
+  k_plain<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, n, size, ctr);
+  return launch_status("plain");
+}
+
+int gal_run_churn(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,
+                  uint64_t* ctr, uint32_t threads, void* stream) {
+  if (!a || !ctr) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
+  k_churn<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, seed, n, rounds, ctr);
+  return launch_status("churn");
+}
+
+int gal_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,
+                     uint32_t lanes_per_address, uint32_t stride, void* stream) {
+  if (!words || span == 0 || lanes_per_address == 0 || stride == 0)
+    return fail(GAL_ERR_INVALID, "bad atomic probe arguments");
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  k_bench_atomic<<<(unsigned)cus * 8, 256, 0, (hipStream_t)stream>>>(words, span, n_ops,
+                                                                      lanes_per_address, stride);
+  return launch_status("bench_atomic");
+}
+
+int gal_bench_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
+  if (!src || !dst || bytes % 16) return fail(GAL_ERR_INVALID, "bad copy arguments");
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  k_bench_copy<<<(unsigned)cus * 8, 256, 0, (hipStream_t)stream>>>(
+      static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
+  return launch_status("bench_copy");
+}
+
+}  // extern "C"

@@ -1,0 +1,687 @@
+// Gallatin device allocator for MI355X (gfx950): device-side malloc / free
+// over segments, blocks of 4096 slices and multi-segment runs.
+//
+// The allocation policy is the reference's (saltsystemslab/gallatin,
+// include/gallatin/allocators/gallatin.cuh): size classes from `smallest` to
+// `biggest` in powers of two, one bitmap tree per class plus the segment
+// tree, 4096-slice blocks claimed by one counter atomic, pinned blocks per
+// class, recycled blocks in a per-segment ring, segments returned when all
+// their blocks are back.  The execution is wave64-native:
+//
+//  * every entry point is called by the active lanes of a wave together; the
+//    lanes are partitioned by size class (malloc) or block (free) with
+//    ballots, every group's counter atomic is issued in one instruction by
+//    the group leaders, and the rare paths (block turnover, segment setup and
+//    release, whole-block and multi-segment requests) run one request at a
+//    time in lane order with the whole wave available for tree searches;
+//  * tree searches scan 64 words per pass across the wave (veb.hpp);
+//  * metadata reads are agent-scope atomics, payload is released on free and
+//    acquired on malloc, so memory moves safely between XCDs.
+//
+// Device API (same names as the reference): Gallatin<S,m,M>::malloc / free /
+// get_allocated_size / owns_allocation; host API: generate_on_device /
+// free_on_device / print_info, implemented over the C ABI in gallatin_capi.h.
+#ifndef GALLATIN_HPP
+#define GALLATIN_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../gallatin_capi.h"
+#include "alloc_utils.hpp"
+#include "layout.hpp"
+#include "murmurhash.hpp"
+#include "veb.hpp"
+
+namespace gallatin {
+namespace allocators {
+namespace detail {
+
+using utils::atomic_add;
+using utils::atomic_and;
+using utils::atomic_cas;
+using utils::atomic_exch;
+using utils::atomic_load;
+using utils::atomic_or;
+using utils::atomic_store;
+using utils::atomic_sub;
+
+enum : uint32_t { kPathSlice = 1, kPathBlock = 2, kPathSegment = 3 };
+
+__device__ __forceinline__ bool deterministic(const gal_state* s) {
+  return (s->flags & kFlagDeterministic) != 0;
+}
+
+// blocks of class t per segment (memory_table.cuh:1057-1061)
+__device__ __forceinline__ uint64_t nb_of(const gal_state* s, uint32_t t) {
+  return s->blocks_per_segment >> t;
+}
+
+// memory_table.cuh:1072-1086 offset_to_allocation, as a byte offset.
+__device__ __forceinline__ uint64_t offset_to_bytes(const gal_state* s,
+                                                    uint64_t off, uint32_t t) {
+  const uint32_t per_seg_shift = s->seg_shift - s->smallest_bits;
+  const uint64_t seg = off >> per_seg_shift;
+  const uint64_t rel = off & ((1ULL << per_seg_shift) - 1);
+  return (seg << s->seg_shift) + (rel << (s->smallest_bits + t));
+}
+
+// memory_table.cuh:1090-1127 allocation_to_offset, from a byte offset.
+__device__ __forceinline__ uint64_t bytes_to_offset(const gal_state* s,
+                                                    uint64_t bytes, uint32_t t) {
+  const uint64_t seg = bytes >> s->seg_shift;
+  const uint64_t in_seg = bytes & ((1ULL << s->seg_shift) - 1);
+  return (in_seg >> (s->smallest_bits + t)) +
+         (seg << (s->seg_shift - s->smallest_bits));
+}
+
+__device__ __forceinline__ void add_stat(gal_state* s, int idx, uint64_t v) {
+  atomic_add(&s->stats[idx], (unsigned long long)v);
+}
+
+// ----------------------------------------------------------- per lane ops
+
+// Segment ring return, gallatin.cuh:1637-1726 return_block with
+// memory_table.cuh:1158-1241 reserve_segment_slot / finish_freeing_block.
+__device__ inline void return_block(gal_state* s, uint32_t blk, uint64_t seg,
+                                    uint32_t t) {
+  const uint64_t nb = nb_of(s, t);
+  gal_seg_queue* q = &s->seg_queue[seg];
+  const uint32_t pos = atomic_add(&q->q_free, 1u);
+  const uint32_t r = pos & (uint32_t)(nb - 1);
+  atomic_exch(&s->ring[(seg << s->bps_shift) + r], blk);
+  utils::fence_release();
+  // finalize_free_queue: tickets complete in order
+  for (uint32_t spin = 0; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
+    if (spin >= kSpinCap) {
+      add_stat(s, kStatErrors, 1);
+      break;
+    }
+    utils::backoff();
+  }
+  // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) in exact integer form
+  if (10ULL * r > nb && r >= 1 && 10ULL * (r - 1) < nb)
+    veb::insert(s->sub_trees[t], seg);
+  const uint64_t old = atomic_add(&s->seg_state[seg], kActiveOne);
+  const int32_t rid = (int32_t)(old >> 32);
+  add_stat(s, kStatBlocksReturned, 1);
+  if ((int64_t)rid == (int64_t)nb - 2) {
+    const uint64_t full = ((uint64_t)(uint32_t)(nb - 1) << 32) | t;
+    const uint64_t closed = (0xFFFFFFFFULL << 32) | t;
+    if (atomic_cas(&s->seg_state[seg], full, closed) == full) {
+      // all blocks back: release the segment (gallatin.cuh:1680-1722)
+      veb::remove(s->sub_trees[t], seg);
+      utils::fence_release();
+      atomic_or(&s->seg_state[seg], (uint64_t)kFreeTree);
+      utils::fence_release();
+      veb::insert(s->seg_tree, seg);
+      add_stat(s, kStatSegmentsReleased, 1);
+    }
+  }
+}
+
+// gallatin.cuh:1732-1766 free_block.
+__device__ inline void free_block(gal_state* s, uint32_t blk) {
+  const uint64_t seg = (uint64_t)blk >> s->bps_shift;
+  const uint32_t t = (uint32_t)(atomic_load(&s->seg_state[seg]) & 0xFFFF);
+  if (t >= s->num_trees) {
+    add_stat(s, kStatErrors, 1);
+    return;
+  }
+  return_block(s, blk, seg, t);
+}
+
+// block.cuh:87-98 block_free_multiple + 200-212 reset_free as used by
+// gallatin.cuh:1774-1827 free_offset.
+__device__ inline void block_release(gal_state* s, uint32_t blk, uint32_t k) {
+  const uint32_t old = atomic_add(&s->blocks[blk].free_counter, k);
+  if (old + k == kSlicesPerBlock) {
+    utils::fence_acquire();
+    atomic_exch(&s->blocks[blk].free_counter, 0u);
+    free_block(s, blk);
+  }
+}
+
+// memory_table.cuh:915-993 get_block.  The packed segment word gives the
+// active count and the owning class in one atomic.
+__device__ inline uint32_t get_block(gal_state* s, uint64_t seg, uint32_t t,
+                                     uint32_t& last) {
+  last = 0;
+  const uint64_t old = atomic_sub(&s->seg_state[seg], kActiveOne);
+  const int32_t ac = (int32_t)(old >> 32);
+  if (ac < 0 || (uint32_t)(old & 0xFFFF) != t) {
+    atomic_add(&s->seg_state[seg], kActiveOne);
+    return kNullBlock;
+  }
+  utils::fence_acquire();
+  const uint64_t nb = nb_of(s, t);
+  const uint32_t q = atomic_add(&s->seg_queue[seg].q_malloc, 1u);
+  uint32_t blk;
+  if ((uint64_t)q < nb) {
+    blk = (uint32_t)((seg << s->bps_shift) + q);
+  } else {
+    blk = atomic_exch(&s->ring[(seg << s->bps_shift) + (q & (uint32_t)(nb - 1))],
+                      kNullBlock);
+  }
+  if (blk == kNullBlock) {  // memory_table.cuh:977-982 traps here
+    add_stat(s, kStatErrors, 1);
+    return kNullBlock;
+  }
+  atomic_exch(&s->blocks[blk].malloc_counter, t << kTreeShift);  // init_malloc
+  last = (ac == 0);
+  return blk;
+}
+
+// gallatin.cuh:1283-1302 submit_segment_for_free: the chunk id is reset
+// before the run goes back to the segment tree, so a concurrent gather can
+// never find the head still marked.
+__device__ inline void free_segments(gal_state* s, uint64_t seg, uint64_t n) {
+  atomic_or(&s->seg_state[seg], (uint64_t)kFreeTree);
+  utils::fence_release();
+  veb::return_multiple(s->seg_tree, seg, n);
+}
+
+// shared_block_storage.cuh:119-134 get_valid_block_index + get_my_block:
+// from `start`, skip empty slots (at most 30 steps); returns the slot's block.
+__device__ inline uint32_t pinned_block(const gal_state* s, uint32_t t,
+                                        uint32_t start, uint32_t& slot) {
+  const uint32_t n = s->slot_count[t];
+  uint32_t* sl = s->slots + s->slot_off[t];
+  const int64_t original = start;
+  int64_t i = start;
+  uint32_t blk = atomic_load(&sl[i]);
+  int counter = 0;
+  while (blk == kNullBlock && i != original - 1) {
+    i = (i + 1) % n;
+    blk = atomic_load(&sl[i]);
+    if (++counter >= kSlotScanCutoff) break;
+  }
+  slot = (uint32_t)i;
+  return blk;
+}
+
+// First slot to try for class t.  Deterministic mode: 0 (as smid 0).  Else
+// the slots of a class are split into 8 groups, one per XCD, and the CU id
+// hashes into its XCD's group, so the counters a CU hits stay in the slots
+// its XCD shares (shared_block_storage.cuh:120 uses smid % slots).
+__device__ __forceinline__ uint32_t slot_start(const gal_state* s, uint32_t t) {
+  if (deterministic(s)) return 0;
+  const uint32_t n = s->slot_count[t];
+  const uint32_t id = __smid();
+  const uint32_t mix = ((id & 63u) * 0x9E3779B1u) >> 26;  // 0..63
+  if (n >= 16) {
+    const uint32_t per = n >> 3;
+    return ((id >> 6) & 7u) * per + mix % per;
+  }
+  return (mix + ((id >> 6) & 7u) * 7u) % n;
+}
+
+// --------------------------------------------------------- wave ops
+// Functions below are called with identical arguments on every active lane
+// and return wave-uniform results.
+
+__device__ __forceinline__ bool try_lock(gal_state* s, uint32_t bit) {
+  uint32_t got = 0;
+  if (wave::is_first()) got = (atomic_or(s->locks, 1u << bit) & (1u << bit)) == 0;
+  return wave::readfirst(got) != 0;
+}
+__device__ __forceinline__ void unlock(gal_state* s, uint32_t bit) {
+  utils::fence_release();
+  if (wave::is_first()) atomic_and(s->locks, ~(1u << bit));
+}
+
+// memory_table.cuh:746-818 setup_segment: claim the chunk id, clear the ring
+// (whole wave), zero the queue counters, then publish nb-1 free blocks.
+__device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t) {
+  uint32_t ok = 0;
+  if (wave::is_first()) {
+    uint64_t cur = atomic_load(&s->seg_state[seg]);
+    for (int g = 0; g < 4096; ++g) {
+      if ((cur & 0xFFFF) != kFreeTree) break;
+      const uint64_t want = (uint64_t)((cur & ~0xFFFFULL) | t);
+      const uint64_t seen = atomic_cas(&s->seg_state[seg], cur, want);
+      if (seen == cur) {
+        ok = 1;
+        break;
+      }
+      cur = seen;
+    }
+  }
+  if (!wave::readfirst(ok)) return false;
+  const uint64_t nb = nb_of(s, t);
+  const uint64_t act = wave::active();
+  const uint32_t k = wave::count(act);
+  const uint32_t rank = wave::count(act & wave::lanes_below());
+  for (uint64_t i = rank; i < nb; i += k)
+    atomic_store(&s->ring[(seg << s->bps_shift) + i], kNullBlock);
+  if (wave::is_first()) {
+    gal_seg_queue* q = &s->seg_queue[seg];
+    atomic_store(&q->q_malloc, 0u);
+    atomic_store(&q->q_free, 0u);
+    atomic_store(&q->q_final, 0u);
+  }
+  utils::fence_release();
+  if (wave::is_first()) {
+    atomic_add(&s->seg_state[seg], nb << 32);  // active -1 -> nb-1
+    add_stat(s, kStatSegmentsSetUp, 1);
+  }
+  return true;
+}
+
+// gallatin.cuh:1465-1510 gather_new_segment: smallest free segment.
+__device__ inline uint64_t gather_new_segment(gal_state* s, uint32_t t) {
+  const uint64_t seg = veb::take_first(s->seg_tree);
+  if (seg == kFail) return kFail;
+  if (!setup_segment(s, seg, t)) {
+    if (wave::is_first()) add_stat(s, kStatErrors, 1);
+    return kFail;
+  }
+  if (wave::is_first()) veb::insert(s->sub_trees[t], seg);
+  return seg;
+}
+
+// Segment of class t to take a block from: smallest id in deterministic
+// mode (find_first_valid_index, veb.cuh:940-944), else from a hashed start
+// (find_random_valid_index, veb.cuh:946-977) wrapping to 0.
+__device__ inline uint64_t find_segment(gal_state* s, uint32_t t, bool det) {
+  const gal_tree& tr = s->sub_trees[t];
+  if (det) return veb::find_ge(tr, 0);
+  const uint64_t gid = wave::readfirst(
+      (uint64_t)threadIdx.x + (uint64_t)blockIdx.x * blockDim.x);
+  const uint64_t start =
+      hashers::MurmurHash64A_u64(gid, s->seed + t + 1) % tr.universe;
+  uint64_t x = veb::find_ge(tr, start);
+  if (x == kFail && start != 0) x = veb::find_ge(tr, 0);
+  return x;
+}
+
+// gallatin.cuh:1525-1631 request_new_block_from_tree.
+__device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, bool det) {
+  int attempts = 0;
+  for (uint32_t spins = 0; attempts < (int)s->lim_request && spins < kSpinCap;
+       ++spins) {
+    uint64_t seg = find_segment(s, t, det);
+    if (seg == kFail) {
+      if (!try_lock(s, t)) {
+        utils::backoff();
+        continue;
+      }
+      seg = gather_new_segment(s, t);
+      unlock(s, t);
+      if (seg == kFail) {
+        ++attempts;
+        continue;
+      }
+    }
+    uint32_t blk = kNullBlock, last = 0;
+    if (wave::is_first()) blk = get_block(s, seg, t, last);
+    blk = wave::readfirst(blk);
+    if (wave::readfirst(last)) {
+      // segment fully handed out: detach it and prefetch the next one
+      if (wave::is_first()) veb::remove(s->sub_trees[t], seg);
+      if (try_lock(s, t)) {
+        gather_new_segment(s, t);
+        unlock(s, t);
+      }
+    }
+    if (blk != kNullBlock) return blk;
+  }
+  return kNullBlock;
+}
+
+// gallatin.cuh:1083-1138 replace_block.
+__device__ inline void replace_block(gal_state* s, uint32_t t, uint32_t slot,
+                                     uint32_t blk, bool det) {
+  uint32_t* sp = s->slots + s->slot_off[t] + slot;
+  uint32_t seen = 0;
+  if (wave::is_first()) seen = atomic_cas(sp, blk, kNullBlock);
+  if (wave::readfirst(seen) != blk) return;
+  const uint32_t nb = request_new_block(s, t, det);
+  if (nb == kNullBlock) return;
+  uint32_t prev = 0;
+  if (wave::is_first()) {
+    utils::fence_release();
+    prev = atomic_cas(sp, kNullBlock, nb);
+    if (prev != kNullBlock) {
+      free_block(s, nb);
+      add_stat(s, kStatErrors, 1);
+    }
+  }
+}
+
+// Slice phase of malloc: gallatin.cuh:865-1078 malloc_slice_allocation for
+// every size class present in the wave at once.
+__device__ inline void malloc_slices(gal_state* s, bool want, uint32_t t,
+                                     uint32_t cnt, uint64_t& out_off) {
+  const bool det = deterministic(s);
+  const uint32_t lane = wave::lane();
+  bool pending = want;
+  uint32_t attempts = 0;
+  for (;;) {
+    const uint64_t pend = __ballot(pending);
+    if (!pend) break;
+    // coalesced groups by class (labeled_partition, gallatin.cuh:883-885)
+    const uint64_t grp = wave::match(pend, pending, t);
+    const int leader = pending ? wave::first(grp) : (int)lane;
+    const bool is_leader = pending && leader == (int)lane;
+    uint32_t slot = 0, blk = kNullBlock;
+    if (is_leader) blk = pinned_block(s, t, slot_start(s, t), slot);
+    slot = wave::fetch(slot, leader);
+    blk = wave::fetch(blk, leader);
+    const bool have = pending && blk != kNullBlock;
+    // exclusive prefix in lane order and group total (block.cuh:245-264)
+    uint32_t prefix, total;
+    if (__ballot(pending && cnt != 1) == 0) {
+      prefix = wave::count(grp & wave::lanes_below());
+      total = wave::count(grp);
+    } else {
+      wave::group_scan(pending ? cnt : 0u, grp, prefix, total);
+    }
+    uint32_t old = 0;
+    if (have && is_leader) old = atomic_add(&s->blocks[blk].malloc_counter, total);
+    old = wave::fetch(old, leader);
+    const uint32_t al = (old & kCountMask) + prefix;
+    const bool start_valid = al <= 4095u;
+    const bool end_valid = al + cnt <= 4096u;
+    const bool should_replace = have && start_valid && (al + cnt > 4095u);
+    uint32_t leftover = 0;
+    if (have && start_valid) leftover = end_valid ? cnt - 1 : 4096u - al;
+    // block_correct_frees (block.cuh:267-281); a credit that completes the
+    // block returns it (the reference leaks such a block)
+    if (__ballot(leftover != 0)) {
+      uint32_t lp, lsum;
+      wave::group_scan(leftover, have ? grp : 0ULL, lp, lsum);
+      bool completes = false;
+      if (have && is_leader && lsum) {
+        const uint32_t of = atomic_add(&s->blocks[blk].free_counter, lsum);
+        completes = of + lsum == kSlicesPerBlock;
+      }
+      for (uint64_t m = __ballot(completes); m; m &= m - 1) {
+        const uint32_t b = wave::readlane(blk, wave::first(m));
+        if (wave::is_first()) {
+          utils::fence_acquire();
+          atomic_exch(&s->blocks[b].free_counter, 0u);
+          free_block(s, b);
+        }
+      }
+    }
+    // the group whose claim covers slice 4095 replaces the block
+    const uint64_t rep = __ballot(should_replace);
+    for (uint64_t m = __ballot(is_leader && (grp & rep) != 0); m; m &= m - 1) {
+      const int l = wave::first(m);
+      replace_block(s, wave::readlane(t, l), wave::readlane(slot, l),
+                    wave::readlane(blk, l), det);
+    }
+    // check_valid (block.cuh:343-357): the block still belongs to class t
+    const bool valid = have && end_valid;
+    const bool ok = valid && (old >> kTreeShift) == t;
+    if (ok) {
+      out_off = (uint64_t)al + ((uint64_t)blk << 12);
+      pending = false;
+    }
+    for (uint64_t m = __ballot(valid && !ok); m; m &= m - 1) {
+      const uint32_t b = wave::readlane(blk, wave::first(m));
+      if (wave::is_first()) block_release(s, b, 1);
+    }
+    if (pending && ++attempts >= s->lim_slice) pending = false;
+  }
+}
+
+// gallatin.cuh:828-861 malloc_block_allocation + block.cuh:288-340
+// malloc_fill_block, retried as gallatin.cuh:1226-1231.  Byte offset.
+__device__ inline uint64_t malloc_block_path(gal_state* s, uint32_t bt, bool det) {
+  for (uint32_t attempt = 0; attempt < s->lim_block; ++attempt) {
+    const uint32_t blk = request_new_block(s, bt, det);
+    if (blk == kNullBlock) continue;
+    uint32_t res = 0;
+    if (wave::is_first()) {
+      const uint32_t merged = atomic_add(&s->blocks[blk].malloc_counter, 4096u);
+      const uint32_t oldc = merged & kCountMask;
+      if ((merged >> kTreeShift) != bt) {  // check_valid -> trap
+        add_stat(s, kStatErrors, 1);
+      } else {
+        atomic_add(&s->blocks[blk].free_counter, 4095u - oldc);
+        if (oldc != 0)
+          block_release(s, blk, 1);
+        else
+          res = 1;
+      }
+    }
+    if (wave::readfirst(res)) return offset_to_bytes(s, (uint64_t)blk << 12, bt);
+  }
+  return kFail;
+}
+
+// gallatin.cuh:790-826 malloc_segment_allocation under the segment lock,
+// retried as gallatin.cuh:1246-1251.  Byte offset.  The lock is taken by one
+// lane of one request at a time, so lanes of a wave never wait on each other.
+__device__ inline uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
+  const uint32_t T = s->num_trees;
+  if (n == 0 || n > s->num_segments || n + T + 1 >= kFreeTree) return kFail;
+  for (uint32_t attempt = 0; attempt < s->lim_segment; ++attempt) {
+    bool got = false;
+    for (uint32_t spin = 0; spin < kSpinCap && !got; ++spin) {
+      got = try_lock(s, T);
+      if (!got) utils::backoff();
+    }
+    if (!got) return kFail;
+    const uint64_t idx = veb::gather_multiple(s->seg_tree, n);
+    uint32_t ok = 0;
+    if (idx != kFail && wave::is_first()) {
+      // set_tree_id (memory_table.cuh:823-828): head marked T + 1 + n
+      uint64_t cur = atomic_load(&s->seg_state[idx]);
+      for (int g = 0; g < 4096; ++g) {
+        if ((cur & 0xFFFF) != kFreeTree) break;
+        const uint64_t seen =
+            atomic_cas(&s->seg_state[idx], cur, (uint64_t)((cur & ~0xFFFFULL) | (T + 1 + n)));
+        if (seen == cur) {
+          ok = 1;
+          break;
+        }
+        cur = seen;
+      }
+      if (!ok) {
+        add_stat(s, kStatErrors, 1);
+        veb::return_multiple(s->seg_tree, idx, n);
+      }
+    }
+    ok = wave::readfirst(ok);
+    unlock(s, T);
+    if (ok) return idx << s->seg_shift;
+  }
+  return kFail;
+}
+
+// gallatin.cuh:1186-1276 malloc.
+__device__ inline void* malloc_impl(gal_state* s, uint64_t size) {
+  const bool det = deterministic(s);
+  const uint32_t T = s->num_trees;
+  const uint32_t lane = wave::lane();
+  // size dispatch (gallatin.cuh:1141-1147, 1196-1258)
+  const int cl = utils::get_first_bit_bigger(size < 1 ? 1 : size);
+  uint32_t t = size < s->smallest ? 0u : (uint32_t)(cl - (int)s->smallest_bits);
+  uint32_t path = kPathSlice, cnt = 1;
+  uint64_t nseg = 0;
+  if (t >= T) {
+    const int bt = cl - (int)s->block_tree_bits;
+    if (bt < 0) {
+      cnt = 1u << (t - (T - 1));
+      t = T - 1;
+    } else if (bt < (int)T) {
+      path = kPathBlock;
+      t = (uint32_t)bt;
+    } else {
+      path = kPathSegment;
+      nseg = (size - 1) / s->bytes_per_segment + 1;
+      t = 0;
+    }
+  }
+  uint64_t bytes = kFail;
+  if (__ballot(path == kPathSlice)) {
+    uint64_t off = kFail;
+    malloc_slices(s, path == kPathSlice, t, cnt, off);
+    if (off != kFail) bytes = offset_to_bytes(s, off, t);
+  }
+  for (uint64_t m = __ballot(path == kPathBlock); m; m &= m - 1) {
+    const int l = wave::first(m);
+    const uint64_t r = malloc_block_path(s, wave::readlane(t, l), det);
+    if ((int)lane == l) bytes = r;
+  }
+  for (uint64_t m = __ballot(path == kPathSegment); m; m &= m - 1) {
+    const int l = wave::first(m);
+    const uint64_t r = malloc_segment_path(s, wave::readlane(nseg, l));
+    if ((int)lane == l) bytes = r;
+  }
+  if (!(s->flags & kFlagNoPayloadFence) && __ballot(bytes != kFail))
+    utils::fence_acquire();
+  return bytes == kFail ? nullptr : (void*)(s->memory + bytes);
+}
+
+// gallatin.cuh:1378-1461 free + 1774-1827 free_offset.
+__device__ inline void free_impl(gal_state* s, void* ptr) {
+  const bool det = deterministic(s);
+  const uint32_t T = s->num_trees;
+  const uint32_t lane = wave::lane();
+  const uint64_t base = (uint64_t)s->memory;
+  const uint64_t p = (uint64_t)ptr;
+  const bool in = ptr != nullptr && (p - base) < (s->num_segments << s->seg_shift);
+  uint64_t seg = 0;
+  uint32_t t = kFreeTree;
+  if (in) {
+    seg = (p - base) >> s->seg_shift;
+    t = (uint32_t)(atomic_load(&s->seg_state[seg]) & 0xFFFF);
+  }
+  const bool multi = in && t > T && t != kFreeTree;
+  const bool slice = in && t < T;
+  const uint64_t bad = __ballot(ptr != nullptr && !multi && !slice);
+  if (bad && wave::is_first()) add_stat(s, kStatErrors, wave::count(bad));
+  if (!__ballot(multi || slice)) return;
+  if (!(s->flags & kFlagNoPayloadFence)) utils::fence_release();
+  // multi-segment frees one at a time in lane order
+  for (uint64_t m = __ballot(multi); m; m &= m - 1) {
+    const int l = wave::first(m);
+    const uint64_t sg = wave::readlane(seg, l);
+    const uint32_t tl = wave::readlane(t, l);
+    if (wave::is_first()) free_segments(s, sg, tl - T - 1);
+  }
+  const uint64_t sm = __ballot(slice);
+  if (!sm) return;
+  uint32_t blk = 0;
+  if (slice) blk = (uint32_t)(bytes_to_offset(s, p - base, t) >> 12);
+  // group lanes freeing into the same block; in deterministic mode all of
+  // them, else the first few groups and single-lane atomics for the rest
+  uint64_t grp = 0, rem = sm;
+  bool grouped = false;
+  const int rounds = det ? 64 : 6;
+  for (int r = 0; rem && r < rounds; ++r) {
+    const uint32_t bl = wave::readlane(blk, wave::first(rem));
+    const bool eq = slice && !grouped && blk == bl;
+    const uint64_t g = __ballot(eq);
+    if (eq) {
+      grp = g;
+      grouped = true;
+    }
+    rem &= ~g;
+  }
+  if (slice && !grouped) grp = 1ULL << lane;
+  const bool is_leader = slice && wave::first(grp) == (int)lane;
+  bool completes = false;
+  if (is_leader) {
+    const uint32_t k = wave::count(grp);
+    const uint32_t old = atomic_add(&s->blocks[blk].free_counter, k);
+    completes = old + k == kSlicesPerBlock;
+  }
+  for (uint64_t m = __ballot(completes); m; m &= m - 1) {
+    const uint32_t b = wave::readlane(blk, wave::first(m));
+    if (wave::is_first()) {
+      utils::fence_acquire();
+      atomic_exch(&s->blocks[b].free_counter, 0u);
+      free_block(s, b);
+    }
+  }
+}
+
+// gallatin.cuh:1152-1182 get_allocated_size.
+__device__ inline uint64_t allocated_size(const gal_state* s, uint64_t bytes) {
+  if (bytes < 16) bytes = 16;
+  const int bit_size = utils::get_first_bit_bigger(bytes);
+  const uint32_t t = bytes < s->smallest ? 0u : (uint32_t)(bit_size - (int)s->smallest_bits);
+  if (t > s->num_trees) {
+    const int block_tree = bit_size - (int)s->block_tree_bits;
+    if (block_tree > (int)s->num_trees) {
+      const uint64_t n = (bytes - 1) / s->bytes_per_segment + 1;
+      return n * s->bytes_per_segment;
+    }
+  }
+  return 1ULL << bit_size;
+}
+
+}  // namespace detail
+
+// Drop-in for gallatin::allocators::Gallatin (gallatin.cuh:292-2035).
+template <uint64_t SegBytes, uint64_t Smallest, uint64_t Biggest>
+struct Gallatin : public gal_state {
+  using my_type = Gallatin<SegBytes, Smallest, Biggest>;
+  static_assert(SegBytes >= Biggest * 4096,
+                "a segment must hold one block of the biggest class");
+  static_assert((SegBytes & (SegBytes - 1)) == 0 && (Smallest & (Smallest - 1)) == 0 &&
+                    (Biggest & (Biggest - 1)) == 0 && Smallest <= Biggest,
+                "sizes must be powers of two");
+
+  // gallatin.cuh:324-421.  Returns nullptr (after printing why) on failure;
+  // calloc mode is not supported.
+  static __host__ my_type* generate_on_device(uint64_t max_bytes, uint64_t seed,
+                                              bool print_info = true,
+                                              bool running_calloc = false) {
+    if (running_calloc) {
+      fprintf(stderr, "Gallatin: calloc mode is not supported\n");
+      return nullptr;
+    }
+    gal_config cfg;
+    gal_config_default(&cfg, max_bytes, seed);
+    cfg.bytes_per_segment = SegBytes;
+    cfg.smallest = Smallest;
+    cfg.biggest = Biggest;
+    cfg.print_info = print_info ? 1u : 0u;
+    gal_allocator* h = nullptr;
+    if (gal_create(&cfg, &h) != GAL_OK) {
+      fprintf(stderr, "Gallatin: generate_on_device failed: %s\n", gal_last_error());
+      return nullptr;
+    }
+    return static_cast<my_type*>(gal_device_state(h));
+  }
+
+  // gallatin.cuh:643-668.
+  static __host__ void free_on_device(my_type* dev_version) {
+    gal_destroy(gal_from_device_state(dev_version));
+  }
+
+  // gallatin.cuh:1855-1897 (called on the device pointer, as in the reference).
+  __host__ void print_info() { gal_print_info(gal_from_device_state(this)); }
+
+  static __host__ __device__ int get_first_bit_bigger(uint64_t x) {
+    return utils::get_first_bit_bigger(x);
+  }
+  static __host__ __device__ int get_num_trees() {
+    return get_first_bit_bigger(Biggest) - get_first_bit_bigger(Smallest) + 1;
+  }
+  static __host__ __device__ uint64_t get_blocks_per_segment(uint16_t tree) {
+    return SegBytes / ((Smallest << tree) * 4096);
+  }
+
+  __device__ void* malloc(uint64_t size) { return detail::malloc_impl(this, size); }
+  __device__ void free(void* ptr) { detail::free_impl(this, ptr); }
+  __device__ uint64_t get_allocated_size(uint64_t bytes) {
+    return detail::allocated_size(this, bytes);
+  }
+  // gallatin.cuh:2027 / memory_table.cuh:1346-1353
+  __device__ bool owns_allocation(void* p) {
+    return ((uint64_t)p - (uint64_t)memory) < (num_segments << seg_shift);
+  }
+};
+
+}  // namespace allocators
+}  // namespace gallatin
+
+#endif

@@ -1,0 +1,122 @@
+// HBM layout of one allocator instance.  Shared by the device code in
+// gallatin.hpp and the host library that creates instances (libgallatin_hip).
+//
+// Reference structures this replaces:
+//   veb_tree / layer        veb.cuh:109-1228   (one cudaMalloc per layer)
+//   alloc_table             memory_table.cuh:229-286
+//   Block                   block.cuh:59-398
+//   pinned_shared_blocks    shared_block_storage.cuh:48-304
+//
+// Differences in layout (all integer, all agent-scope atomics):
+//   * all tree levels of all trees live in one arena of 64-bit words; a level
+//     of w words is scanned by the wave 64 words per pass;
+//   * the per-segment chunk id (u16) and active count (int) are packed into
+//     one 64-bit word `seg_state` = (active << 32) | tree_id, so `free` reads
+//     one aligned word per pointer and `get_block` learns count and owner in
+//     one atomic;
+//   * ring entries and pinned slots hold 32-bit block ids instead of Block*.
+#ifndef GALLATIN_LAYOUT_HPP
+#define GALLATIN_LAYOUT_HPP
+
+#include <stdint.h>
+
+namespace gallatin {
+namespace allocators {
+
+constexpr uint32_t kSlicesPerBlock = 4096;   // block.cuh:84-97
+constexpr uint32_t kTreeShift = 20;          // block.cuh:53
+constexpr uint32_t kCountMask = (1u << kTreeShift) - 1u;
+constexpr uint32_t kFreeTree = 0xFFFFu;      // memory_table.cuh:485
+constexpr uint32_t kNullBlock = 0xFFFFFFFFu;
+constexpr uint64_t kFail = ~0ULL;
+constexpr int kMaxLayers = 4;                // universes up to 2^24 segments
+constexpr int kMaxTrees = 24;
+
+// retry limits (gallatin.cuh:108-112, shared_block_storage.cuh:31)
+constexpr int kRequestBlockAttempts = 1000;
+constexpr int kSliceAttempts = 500 * 10;
+constexpr int kBlockAttempts = 500;
+constexpr int kSegmentAttempts = 500;
+constexpr int kSlotScanCutoff = 30;
+// bound on loops the reference leaves unbounded (lost lock races)
+constexpr uint32_t kSpinCap = 1u << 16;
+
+// flags
+constexpr uint32_t kFlagDeterministic = 1u;  // smallest-id segment, slot 0
+constexpr uint32_t kFlagNoPayloadFence = 2u; // skip release/acquire on payload
+
+// seg_state packing
+constexpr uint64_t kActiveOne = 1ULL << 32;
+constexpr uint64_t kSegFreeState = 0xFFFFFFFF0000FFFFULL;  // active -1, tree free
+
+struct gal_block {
+  uint32_t malloc_counter;  // (tree << 20) | claimed slices
+  uint32_t free_counter;    // freed + pre-credited slices
+};
+
+struct gal_tree {
+  uint64_t* words;                     // layer 0 (top) .. num_layers-1 (bottom)
+  uint64_t universe;                   // bits in the bottom layer
+  uint32_t num_layers;
+  uint32_t pad;
+  uint64_t layer_off[kMaxLayers];      // word offset of each layer in `words`
+  uint64_t layer_words[kMaxLayers];
+};
+
+struct gal_seg_queue {
+  uint32_t q_malloc;  // ring read position   (memory_table.cuh:244)
+  uint32_t q_free;    // ring write ticket    (memory_table.cuh:247)
+  uint32_t q_final;   // ring write completion (memory_table.cuh:249)
+  uint32_t pad;
+};
+
+// stats[] indices
+enum gal_stat {
+  kStatErrors = 0,      // invalid frees and internal inconsistencies
+  kStatBootFailed = 1,
+  kStatSegmentsSetUp = 2,
+  kStatSegmentsReleased = 3,
+  kStatBlocksReturned = 4,
+  kStatCount = 8
+};
+
+struct gal_state {
+  // geometry, fixed at creation
+  uint64_t bytes_per_segment;
+  uint64_t smallest;
+  uint64_t biggest;
+  uint64_t num_segments;
+  uint64_t blocks_per_segment;   // S / (smallest * 4096)
+  uint64_t seed;
+  uint32_t seg_shift;            // log2(bytes_per_segment)
+  uint32_t smallest_bits;        // log2(smallest)
+  uint32_t bps_shift;            // log2(blocks_per_segment)
+  uint32_t num_trees;
+  uint32_t block_tree_bits;      // log2(smallest * 4096)
+  uint32_t flags;
+  uint32_t total_slots;
+  uint32_t pad0;
+  // retry limits (defaults kRequestBlockAttempts, kSliceAttempts, ...)
+  uint32_t lim_request;
+  uint32_t lim_slice;
+  uint32_t lim_block;
+  uint32_t lim_segment;
+  // storage
+  char* memory;                  // num_segments * bytes_per_segment
+  uint64_t* seg_state;           // [N]
+  gal_seg_queue* seg_queue;      // [N]
+  uint32_t* ring;                // [N * B] recycled block ids
+  gal_block* blocks;             // [N * B]
+  uint32_t* slots;               // pinned blocks of all size classes
+  uint32_t* locks;               // [1] tree lock bits (gallatin.cuh:314)
+  unsigned long long* stats;     // [kStatCount]
+  uint32_t slot_off[kMaxTrees + 1];
+  uint32_t slot_count[kMaxTrees];
+  gal_tree seg_tree;
+  gal_tree sub_trees[kMaxTrees];
+};
+
+}  // namespace allocators
+}  // namespace gallatin
+
+#endif

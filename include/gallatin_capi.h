@@ -1,0 +1,198 @@
+/*
+ * C ABI of libgallatin_hip.so: host lifecycle of device allocators and the
+ * workload kernels that drive the device malloc/free path.
+ *
+ * The reference (saltsystemslab/gallatin) is header-only CUDA C++ with no FFI;
+ * its host API is the static/host members of Gallatin<> and the free
+ * functions of global_allocator.cuh.  Each entry point below names the
+ * reference interface it replaces (paths relative to the reference checkout,
+ * include/gallatin/allocators/ unless stated otherwise).  Device-side
+ * malloc/free stay header-inline (include/gallatin/allocators/gallatin.hpp),
+ * as in the reference, because device code cannot cross a C ABI.
+ *
+ * Conventions: every function returns GAL_OK (0) or a negative gal_status;
+ * nothing calls exit().  Pointers named *_dev are device pointers; `stream`
+ * is a hipStream_t (NULL = default stream).  Workload entry points enqueue
+ * work on `stream` and return without synchronising unless stated.
+ */
+#ifndef GALLATIN_CAPI_H
+#define GALLATIN_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gal_status {
+  GAL_OK = 0,
+  GAL_ERR_INVALID = -1,     /* bad argument or geometry */
+  GAL_ERR_HIP = -2,         /* a HIP runtime call failed */
+  GAL_ERR_BOOT = -3,        /* pool too small to pin the boot blocks */
+  GAL_ERR_UNSUPPORTED = -4, /* e.g. calloc mode */
+  GAL_ERR_NO_DEVICE = -5
+} gal_status;
+
+/* flags */
+#define GAL_FLAG_DETERMINISTIC 1u     /* smallest-id segments, slot scan from 0 */
+#define GAL_FLAG_NO_PAYLOAD_FENCE 2u  /* caller orders payload itself */
+
+typedef struct gal_config {
+  uint64_t max_bytes;          /* pool bytes; segments = max_bytes / bytes_per_segment */
+  uint64_t bytes_per_segment;  /* power of two, >= biggest * 4096 */
+  uint64_t smallest;           /* smallest slice, power of two */
+  uint64_t biggest;            /* biggest slice, power of two */
+  uint64_t seed;
+  uint32_t flags;
+  uint32_t slots0;             /* pinned blocks of the smallest class (reference 128) */
+  uint32_t slots_min;          /* floor of the halving slot counts (reference 4) */
+  uint32_t print_info;         /* print the boot line, as generate_on_device does */
+  /* retry limits, 0 = the reference's (gallatin.cuh:108-112): block
+   * requests 1000, slice claims 500 x 10, whole blocks 500, segment runs 500 */
+  uint32_t request_attempts;
+  uint32_t slice_attempts;
+  uint32_t block_attempts;
+  uint32_t segment_attempts;
+} gal_config;
+
+typedef struct gal_allocator gal_allocator;
+
+typedef struct gal_info {
+  uint64_t num_segments, bytes_per_segment, smallest, biggest;
+  uint64_t blocks_per_segment;
+  uint32_t num_trees, total_slots;
+  uint64_t segments_free;          /* bits set in the segment tree */
+  uint64_t segments_formatted;     /* segments owned by a size class */
+  uint64_t segments_multi;         /* heads of multi-segment allocations */
+  uint64_t tree_owned[24];         /* bits set in each size-class tree */
+  uint64_t errors;                 /* invalid frees / internal inconsistencies */
+  uint64_t overhead_bytes;         /* metadata bytes (calculate_overhead) */
+  uint64_t pool_base;              /* device address of the pool */
+} gal_info;
+
+/* Fill `cfg` with the reference defaults: Gallatin<16 MiB, 16, 4096>,
+ * 128/4 pinned slots (gallatin.cuh:346, 124; global_allocator.cuh:41). */
+void gal_config_default(gal_config *cfg, uint64_t max_bytes, uint64_t seed);
+
+/* Gallatin<S,m,M>::generate_on_device (gallatin.cuh:324-421): allocate the
+ * pool and metadata on the current device, boot the pinned blocks and
+ * return a handle.  Synchronous. */
+int gal_create(const gal_config *cfg, gal_allocator **out);
+
+/* Gallatin<S,m,M>::free_on_device (gallatin.cuh:643-668). */
+int gal_destroy(gal_allocator *a);
+
+/* Device address of the allocator object, usable as Gallatin<S,m,M>* in
+ * kernels (the pointer generate_on_device returns). */
+void *gal_device_state(gal_allocator *a);
+
+/* Handle owning a device allocator object, or NULL (lets the C++ wrapper
+ * serve print_info() called on a device pointer, gallatin.cuh:1855). */
+gal_allocator *gal_from_device_state(const void *dev_state);
+
+/* print_info (gallatin.cuh:1855-1897): segment-tree fill, per-class owned
+ * segments, overhead.  Synchronous. */
+int gal_print_info(gal_allocator *a);
+
+/* Statistics behind print_info as numbers.  Synchronous. */
+int gal_get_info(gal_allocator *a, gal_info *out);
+
+/* Quiescent drain (no counterpart in the reference): give back every pinned
+ * block's unclaimed slices and release formatted segments whose blocks are
+ * all free, so that after all frees the metadata equals a fresh instance.
+ * Pinned slots are empty afterwards; call gal_reboot before new work. */
+int gal_drain(gal_allocator *a, void *stream);
+
+/* Re-pin boot blocks after a drain (boot_shared_block_container order). */
+int gal_reboot(gal_allocator *a, void *stream);
+
+/* Copy metadata to host buffers for checking (any pointer may be NULL):
+ * seg_state[N] (active << 32 | tree id), slots[total_slots], blocks[2*N*B]
+ * (malloc_counter, free_counter).  Synchronous. */
+int gal_snapshot(gal_allocator *a, uint64_t *seg_state, uint32_t *slots,
+                 uint32_t *blocks);
+
+/* Tree words: tree -1 = segment tree, 0..T-1 = size-class trees.  Writes
+ * the words of all layers, top layer first, and their counts.  Returns the
+ * number of words needed when out is NULL. */
+int64_t gal_tree_words(gal_allocator *a, int tree, uint64_t *out, uint64_t cap,
+                       uint32_t *num_layers, uint64_t *layer_words);
+
+/* ---- global instance (global_allocator.cuh:41-97) ---- */
+/* init_global_allocator / free_global_allocator / print_global_stats for the
+ * library's own global_gallatin symbol (Gallatin<16 MiB, 16, 4096>). */
+int gal_init_global(uint64_t num_bytes, uint64_t seed, int print_info);
+int gal_free_global(void);
+int gal_print_global_stats(void);
+gal_allocator *gal_global_handle(void);
+
+/* ---- workload kernels (the reference's test kernels) ---- */
+
+/* One thread per request: out_dev[i] = allocator->malloc(sizes_dev[i]).
+ * tests/src/gallatin_test.cu:59-86 alloc_one_size_pointer, variable sizes. */
+int gal_malloc_bulk(gal_allocator *a, const uint64_t *sizes_dev, uint64_t n,
+                    void **out_dev, uint32_t block_threads, void *stream);
+
+/* One thread per pointer: allocator->free(ptrs_dev[i]) (NULL skipped).
+ * tests/src/gallatin_test.cu:90-112 free_one_size_pointer. */
+int gal_free_bulk(gal_allocator *a, void *const *ptrs_dev, uint64_t n,
+                  uint32_t block_threads, void *stream);
+
+/* Same through global_malloc / global_free (global_allocator.cuh:72-82). */
+int gal_global_malloc_bulk(const uint64_t *sizes_dev, uint64_t n,
+                           void **out_dev, uint32_t block_threads, void *stream);
+int gal_global_free_bulk(void *const *ptrs_dev, uint64_t n,
+                         uint32_t block_threads, void *stream);
+
+/* Single-wave deterministic stream (one 64-lane wave).  Step layout: u32
+ * kind (0 malloc, 1 free), u32 pad, u64 lane mask, u64 arg[64]; a malloc arg
+ * is a size, a free arg is the index step*64+lane of an earlier malloc.
+ * results_dev[step*64+lane] = byte offset from the pool base or ~0. */
+typedef struct gal_wave_step {
+  uint32_t kind;
+  uint32_t pad;
+  uint64_t mask;
+  uint64_t arg[64];
+} gal_wave_step;
+int gal_run_wave_stream(gal_allocator *a, const gal_wave_step *steps_dev,
+                        uint64_t nsteps, uint64_t *results_dev, void *stream);
+
+/* Counters written by the workload kernels (device array, 8 x u64):
+ * [0] pairs (successful malloc+free), [1] misses (nullptr), [2] pattern
+ * mismatches, [3] payload bytes written+read, [4] mallocs issued. */
+enum { GAL_CTR_PAIRS = 0, GAL_CTR_MISSES = 1, GAL_CTR_BAD = 2,
+       GAL_CTR_BYTES = 3, GAL_CTR_MALLOCS = 4, GAL_CTR_COUNT = 8 };
+
+/* Plain method: nthreads threads each malloc(size), write their uint32 id,
+ * read it back and free (config "plain small"; gallatin_test.cu:59-112 in
+ * one kernel). */
+int gal_run_plain(gal_allocator *a, uint64_t nthreads, uint64_t size,
+                  uint64_t *counters_dev, uint32_t block_threads, void *stream);
+
+/* Mixed-size churn: nthreads threads x rounds of streams.h's churn with one
+ * live allocation per thread, every allocation filled with its (thread,
+ * round) pattern and verified before free (gallatin_churn.cu:37-95 with
+ * log-uniform sizes). */
+int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
+                  uint32_t rounds, uint64_t *counters_dev,
+                  uint32_t block_threads, void *stream);
+
+/* ---- measurement kernels (ceilings for the roofline) ---- */
+/* n_ops relaxed agent-scope atomicAdd(u32) with the returned value kept:
+ * op i targets word ((i / lanes_per_address) * stride) % span, so
+ * lanes_per_address = 64 puts one address per wave and span = 1 one address
+ * for the whole device. */
+int gal_bench_atomic(uint32_t *words_dev, uint64_t span, uint64_t n_ops,
+                     uint32_t lanes_per_address, uint32_t stride, void *stream);
+/* Copy of `bytes` (multiple of 16) with 16-byte loads and stores. */
+int gal_bench_copy(const void *src_dev, void *dst_dev, uint64_t bytes,
+                   void *stream);
+
+/* Last error message of the calling thread (static storage). */
+const char *gal_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

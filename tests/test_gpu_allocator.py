@@ -1,0 +1,202 @@
+"""Parity of the HIP allocator with the oracle (GPU).
+
+Deterministic single-wave streams must return exactly the oracle's offsets
+and leave bit-identical metadata; concurrent runs at full size must satisfy
+the exact invariants (in pool, class-aligned, disjoint, patterns intact,
+metadata back to the fresh state after all frees and a drain).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+G = H.golden()
+S = 16 << 20
+# single-wave runs: once an attempt has failed without changing state every
+# later attempt fails the same way, so small limits give the same results
+QUICK = (6, 6, 6, 6)
+
+
+def make(gpu, pool, det=True, limits=QUICK, seed=42):
+    flags = gpu.GAL_FLAG_DETERMINISTIC if det else 0
+    return gpu.Gallatin(S, 16, 4096).generate_on_device(
+        pool, seed, print_info=False, flags=flags, limits=limits)
+
+
+def compare_metadata(a, o):
+    """GPU instance `a` and oracle `o` hold identical metadata."""
+    st, sl, bl = a.snapshot()
+    ids, act = o.segment_state()
+    assert np.array_equal(st, H.pack_state(ids, act)), "segment state differs"
+    assert np.array_equal(sl, o.slots()), "pinned slots differ"
+    assert np.array_equal(bl, o.blocks()), "block counters differ"
+    assert all(np.array_equal(x, y) for x, y in
+               zip(a.tree_layers(-1), o.segment_tree().layers())), "segment tree differs"
+    for t in range(a.num_trees):
+        assert all(np.array_equal(x, y) for x, y in
+                   zip(a.tree_layers(t), o.sub_tree(t).layers())), f"class tree {t} differs"
+
+
+def check_fresh(a):
+    """Metadata of an instance with nothing allocated and nothing pinned."""
+    st, sl, bl = a.snapshot()
+    assert np.all(st == np.uint64(0xFFFFFFFF0000FFFF))
+    assert np.all(sl == 0xFFFFFFFF)
+    assert np.all(bl[:, 1] == 0), "a block kept freed slices"
+    for x, y in zip(a.tree_layers(-1), H.fresh_layers(a.num_segments, True)):
+        assert np.array_equal(x, y)
+    for t in range(a.num_trees):
+        for x in a.tree_layers(t):
+            assert not np.any(x)
+    assert a.info()["errors"] == 0
+
+
+def test_boot_matches_oracle_and_golden(gpu):
+    g = G["boot_1GiB_16MiB_16_4096"]
+    a = make(gpu, g["pool_bytes"])
+    o = oracle.Allocator(g["pool_bytes"], limits=QUICK)
+    compare_metadata(a, o)
+    st, _, _ = a.snapshot()
+    assert (st[:16] & np.uint64(0xFFFF)).tolist() == g["chunk_ids_first_16"]
+    a.free_on_device()
+
+
+def _single_lane_steps(sizes):
+    steps = np.zeros(len(sizes), H.STEP_DTYPE)
+    for i, s in enumerate(sizes):
+        steps[i]["kind"] = 0
+        steps[i]["mask"] = 1
+        steps[i]["arg"][0] = s
+    return steps
+
+
+def test_single_lane_known_answers(gpu):
+    seq = G["single_lane_sequence_1GiB"]
+    a = make(gpu, 1 << 30)
+    res = a.run_wave_stream(_single_lane_steps([s["size"] for s in seq]))
+    got = [None if int(res[i * 64]) == H.FAIL else int(res[i * 64]) for i in range(len(seq))]
+    assert got == [s["offset"] for s in seq]
+    o = oracle.Allocator(1 << 30, limits=QUICK)
+    o.run_stream(_single_lane_steps([s["size"] for s in seq]))
+    compare_metadata(a, o)
+    a.free_on_device()
+
+
+def test_wave_known_answer(gpu):
+    g = G["wave_of_64_x16_after_boot_1GiB"]
+    a = make(gpu, 1 << 30)
+    steps = np.zeros(1, H.STEP_DTYPE)
+    steps[0]["mask"] = (1 << 64) - 1
+    steps[0]["arg"][:] = 16
+    res = a.run_wave_stream(steps)
+    assert res[:64].tolist() == [g["first"] + g["stride"] * i for i in range(64)]
+    a.free_on_device()
+
+
+def test_multi_segment_known_answers(gpu):
+    seq = G["multi_segment_4GiB"]
+    a = make(gpu, 4 << 30)
+    res = a.run_wave_stream(_single_lane_steps([s["size"] for s in seq]))
+    assert [int(res[i * 64]) for i in range(len(seq))] == [s["offset"] for s in seq]
+    a.free_on_device()
+
+
+@pytest.mark.parametrize("seed,pool,mix,nsteps", [
+    (1, 1 << 30, "slices", 400),
+    (2, 4 << 30, "all", 400),
+    (3, 256 << 20, "all", 300),      # small pool: exhaustion and recycling
+    (4, 64 << 30, "all", 300),       # 3-level trees
+])
+def test_stream_parity(gpu, seed, pool, mix, nsteps):
+    steps, live = H.make_stream(seed, nsteps, mix=mix)
+    steps = np.concatenate([steps, H.free_all_steps(live, len(steps))])
+    o = oracle.Allocator(pool, limits=QUICK)
+    exp = o.run_stream(steps)
+    a = make(gpu, pool)
+    got = a.run_wave_stream(steps)
+    if not np.array_equal(got, exp):
+        bad = np.nonzero(got != exp)[0]
+        i = int(bad[0])
+        pytest.fail(f"first difference at step {i // 64} lane {i % 64}: "
+                    f"gpu {int(got[i]):#x} oracle {int(exp[i]):#x} ({len(bad)} differ)")
+    compare_metadata(a, o)
+    a.drain()
+    o.drain()
+    compare_metadata(a, o)
+    check_fresh(a)
+    a.free_on_device()
+
+
+def test_plain_small_config(gpu):
+    """1 GiB pool, 2^20 threads x 10 launches of malloc(16) / write id /
+    read back / free; every pair succeeds and the pool returns to fresh."""
+    import torch
+    a = make(gpu, 1 << 30, det=False, limits=None)
+    ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
+    for _ in range(10):
+        a.run_plain(1 << 20, 16, counters=ctr)
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    assert c[gpu.CTR_PAIRS] == 10 << 20
+    assert c[gpu.CTR_MISSES] == 0 and c[gpu.CTR_BAD] == 0
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
+def test_bulk_live_set_disjoint(gpu):
+    """2^18 mixed requests live at once: in pool, aligned, disjoint; then all
+    freed (random order) and drained back to the fresh state."""
+    import torch
+    rng = np.random.default_rng(5)
+    n = 1 << 18
+    sizes = H.random_sizes(rng, n, "slices")
+    a = make(gpu, 8 << 30, det=False, limits=None)
+    d_sizes = torch.from_numpy(sizes.astype(np.int64)).cuda()
+    ptrs = a.malloc_bulk(d_sizes)
+    torch.cuda.synchronize()
+    p = ptrs.cpu().numpy().view(np.uint64)
+    assert np.all(p != 0), "unexpected nullptr"
+    H.check_ptrs(p, sizes, a.pool_base, a.num_segments * S)
+    perm = torch.from_numpy(rng.permutation(n)).cuda()
+    a.free_bulk(ptrs[perm].contiguous())
+    torch.cuda.synchronize()
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
+def test_churn_patterns_and_counts(gpu):
+    import torch
+    n, rounds, seed = 1 << 20, 10, 9
+    a = make(gpu, 8 << 30, det=False, limits=None)
+    ctr = a.run_churn(seed, n, rounds)
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    assert c[gpu.CTR_BAD] == 0
+    assert c[gpu.CTR_MISSES] == 0
+    assert c[gpu.CTR_PAIRS] == H.expected_churn_pairs(seed, n, rounds)
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
+def test_global_allocator(gpu):
+    import torch
+    gpu.init_global_allocator(1 << 30, 42, print_info=False)
+    sizes = torch.full((1 << 16,), 64, dtype=torch.int64, device="cuda")
+    ptrs = gpu.global_malloc_bulk(sizes)
+    torch.cuda.synchronize()
+    p = ptrs.cpu().numpy().view(np.uint64)
+    assert np.all(p != 0)
+    g = gpu.global_allocator()
+    H.check_ptrs(p, np.full(len(p), 64, np.uint64), g.pool_base, g.num_segments * S)
+    gpu.global_free_bulk(ptrs)
+    torch.cuda.synchronize()
+    gpu.print_global_stats()
+    g.drain()
+    check_fresh(g)
+    gpu.free_global_allocator()
