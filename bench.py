@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark: device malloc+free pairs/s on the mixed-size churn, one MI355X.
+
+Workload (one step = one pass of the hot path over one batch):
+  mixed-size churn, 8 GiB pool, Gallatin<16 MiB, 16, 4096>, 2^24 threads x 10
+  rounds; sizes log-uniform in [1, 4096) B; each round a thread replaces its
+  live allocation with probability 1/2; every allocation is filled with a
+  uint64 (thread, round) pattern and verified before it is freed.
+
+Prints one JSON line (rank 0).  `value` is successful malloc+free pairs per
+second over the timed steps, counted on the device; the roofline is that of
+the churn kernel (payload bytes written and read against HBM peak, and the
+atomic issue rate against the measured atomic ceiling); `cpu_baseline` is the
+oracle's sequential replay of the same request streams on one host core, with
+multithreaded glibc malloc/free of the same streams beside it.
+
+  python bench.py [--steps K] [--warmup W] [--threads N] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("malloc+free pairs/sec (mixed 16B–4KiB churn, 1 GPU) batched vs plain; "
+          "p99 step")
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--threads", type=int, default=1 << 24)
+    p.add_argument("--rounds", type=int, default=10)
+    p.add_argument("--pool-gib", type=int, default=8)
+    p.add_argument("--seed", type=int, default=1234)
+    p.add_argument("--slots0", type=int, default=128)
+    p.add_argument("--slots-min", type=int, default=4)
+    p.add_argument("--flags", type=int, default=0)
+    p.add_argument("--block-threads", type=int, default=256)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-ceilings", action="store_true")
+    p.add_argument("--cpu-sample-threads", type=int, default=3 << 19)
+    return p.parse_args()
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def atomic_ceiling(G, torch):
+    """Agent-scope atomicAdd(u32) with returned value: distinct consecutive
+    addresses, one address per 128-byte line, one address per wave."""
+    words = torch.zeros(1 << 26, dtype=torch.int32, device="cuda")
+    out = {}
+    for name, span, lanes, stride, nops in (
+            ("distinct", 1 << 26, 1, 1, 1 << 28),
+            ("per_line", 1 << 26, 1, 32, 1 << 27),
+            ("per_wave", 1 << 26, 64, 32, 1 << 26)):
+        G.bench_atomic(words, span, nops, lanes, stride)  # warm
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        G.bench_atomic(words, span, nops, lanes, stride)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = nops / (e0.elapsed_time(e1) / 1e3)
+    src = torch.empty(1 << 31, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    G.bench_copy(src, dst, src.numel())
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        G.bench_copy(src, dst, src.numel())
+    e1.record()
+    torch.cuda.synchronize()
+    out["copy_gbs"] = 5 * 2 * src.numel() / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del words, src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baselines(args):
+    import oracle
+    res = {}
+    a = oracle.Allocator(args.pool_gib << 30)
+    n = args.cpu_sample_threads
+    t0 = time.perf_counter()
+    pairs, misses, bad = a.run_churn_sequential(args.seed, 0, n, args.rounds)
+    dt = time.perf_counter() - t0
+    a.close()
+    res["cpu_baseline"] = {
+        "value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+        "sample": f"oracle sequential replay of stream threads 0..{n - 1} "
+                  f"({pairs} pairs, {args.rounds} rounds, fill+verify) in {dt:.1f} s",
+    }
+    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
+
+    class R(ctypes.Structure):
+        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
+                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
+                    ("pad", ctypes.c_uint32)]
+    lib.gal_cpu_churn_glibc.argtypes = [ctypes.c_uint64] * 3 + [
+        ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(R)]
+    r = R()
+    cores = host_cores()
+    lib.gal_cpu_churn_glibc(args.seed, 0, args.threads, args.rounds, cores, ctypes.byref(r))
+    res["cpu_glibc"] = {"value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+                        "kind": "glibc malloc/free",
+                        "sample": f"all {args.threads} stream threads, {r.pairs} pairs "
+                                  f"in {r.seconds:.2f} s, pattern errors {r.bad}"}
+    return res
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py measures one GPU: the allocator is a per-GPU instance and this "
+              "build runs on a single MI355X; --gpus above 1 is not supported.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gallatin_amd as G
+
+    torch.cuda.set_device(0)
+    alloc = G.Gallatin(16 << 20, 16, 4096).generate_on_device(
+        args.pool_gib << 30, args.seed, print_info=False, flags=args.flags,
+        slots0=args.slots0, slots_min=args.slots_min)
+    ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
+    for w in range(args.warmup):
+        alloc.run_churn(args.seed + w, args.threads, args.rounds, counters=ctr,
+                        block_threads=args.block_threads)
+    torch.cuda.synchronize()
+    ctr.zero_()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record()
+    for k in range(args.steps):
+        alloc.run_churn(args.seed + 100 + k, args.threads, args.rounds, counters=ctr,
+                        block_threads=args.block_threads)
+        evs[k + 1].record()
+    torch.cuda.synchronize()
+    total_ms = evs[0].elapsed_time(evs[-1])
+    step_ms = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
+    c = ctr.cpu().tolist()
+    info = alloc.info()
+    pairs, misses, bad, nbytes = c[G.CTR_PAIRS], c[G.CTR_MISSES], c[G.CTR_BAD], c[G.CTR_BYTES]
+    value = pairs / (total_ms / 1e3)
+    avg_ms = total_ms / args.steps
+    achieved_gbs = nbytes / args.steps / (avg_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": avg_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (counter-based request streams generated on the device)",
+        "config": {
+            "workload": "mixed-size churn: 8 GiB pool, 2^24 threads x 10 rounds, "
+                        "log-uniform 1..4095 B, 50% replace per round, fill+verify",
+            "path": "plain per-thread malloc/free",
+            "allocator": "Gallatin<16 MiB, 16 B, 4096 B>",
+            "pool_bytes": args.pool_gib << 30,
+            "threads": args.threads,
+            "rounds": args.rounds,
+            "slots0": args.slots0,
+            "slots_min": args.slots_min,
+            "block_threads": args.block_threads,
+        },
+        "p50_step_ms": step_ms[len(step_ms) // 2],
+        "p99_step_ms": step_ms[min(len(step_ms) - 1, int(round(0.99 * (len(step_ms) - 1))))],
+        "pairs_per_step": pairs / args.steps,
+        "misses": misses,
+        "violations": bad + info["errors"],
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_churn",
+            "algorithmic_bytes_per_launch": nbytes / args.steps,
+        },
+    }
+    alloc.free_on_device()
+    if not args.no_ceilings:
+        ceil = atomic_ceiling(G, torch)
+        atoms = 2.0 * value  # one claim + one release per pair
+        out["atomic_roofline"] = {
+            "achieved": atoms, "peak": ceil["distinct"], "unit": "atomics/s",
+            "frac": atoms / ceil["distinct"], "per_pair": 2,
+            "ceilings": ceil,
+        }
+    if not args.no_cpu_baseline:
+        out.update(cpu_baselines(args))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

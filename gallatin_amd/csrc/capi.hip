@@ -55,7 +55,7 @@ __global__ void k_init_segments(gal_state* s) {
 __global__ void k_init_blocks(gal_state* s) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (s->num_segments << s->bps_shift)) return;
-  s->blocks[i] = gal_block{4097u, 0u};  // Block::init, block.cuh:64-68
+  *detail::block_at(s, (uint32_t)i) = gal_block{4097u, 0u};  // Block::init, block.cuh:64-68
   s->ring[i] = kNullBlock;
 }
 
@@ -99,7 +99,7 @@ __global__ void k_drain(gal_state* s) {
       if (wave::is_first()) {
         const uint32_t blk = utils::atomic_exch(&s->slots[s->slot_off[t] + i], kNullBlock);
         if (blk != kNullBlock) {
-          uint32_t c = utils::atomic_load(&s->blocks[blk].malloc_counter) & kCountMask;
+          uint32_t c = utils::atomic_load(&detail::block_at(s, blk)->malloc_counter) & kCountMask;
           if (c > 4096u) c = 4096u;
           if (c < 4096u) detail::block_release(s, blk, 4096u - c);
         }
@@ -260,7 +260,8 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   const uint64_t o_seg = take(N * sizeof(uint64_t));
   const uint64_t o_queue = take(N * sizeof(gal_seg_queue));
   const uint64_t o_ring = take(N * B * sizeof(uint32_t));
-  const uint64_t o_blocks = take(N * B * sizeof(gal_block));
+  h.block_bits = (N * B) <= 1 ? 0 : 64u - (uint32_t)__builtin_clzll(N * B - 1);
+  const uint64_t o_blocks = take((1ULL << h.block_bits) * sizeof(gal_block));
   const uint64_t o_slots = take((uint64_t)total * sizeof(uint32_t));
   const uint64_t o_locks = take(256);
   const uint64_t o_stats = take(kStatCount * sizeof(unsigned long long));
@@ -385,8 +386,18 @@ int gal_snapshot(gal_allocator* a, uint64_t* seg_state, uint32_t* slots, uint32_
     return hip_fail(e, "snapshot seg_state");
   if (slots && (e = hipMemcpy(slots, h.slots, (size_t)h.total_slots * 4, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_fail(e, "snapshot slots");
-  if (blocks && (e = hipMemcpy(blocks, h.blocks, (h.num_segments << h.bps_shift) * 8, hipMemcpyDeviceToHost)) != hipSuccess)
-    return hip_fail(e, "snapshot blocks");
+  if (blocks) {
+    // gather block counters back into block-id order
+    std::vector<gal_block> raw(1ULL << h.block_bits);
+    if ((e = hipMemcpy(raw.data(), h.blocks, raw.size() * sizeof(gal_block), hipMemcpyDeviceToHost)) != hipSuccess)
+      return hip_fail(e, "snapshot blocks");
+    const uint64_t nblk = h.num_segments << h.bps_shift;
+    for (uint64_t i = 0; i < nblk; ++i) {
+      const gal_block& b = raw[block_position(i, h.block_bits)];
+      blocks[2 * i] = b.malloc_counter;
+      blocks[2 * i + 1] = b.free_counter;
+    }
+  }
   return GAL_OK;
 }
 

@@ -25,25 +25,67 @@ __device__ inline uint64_t wave_sum(uint64_t v) {
   return v;
 }
 
-__device__ inline void add_counters(uint64_t* ctr, const uint64_t* v, int n) {
-  for (int i = 0; i < n; ++i) {
-    const uint64_t s = wave_sum(v[i]);
-    if (wave::lane() == 0 && s) utils::atomic_add(&ctr[i], (uint64_t)s);
+// Per-lane counts of one thread, summed over the wave and added with one
+// atomic per counter per wave.
+struct lane_counts {
+  uint32_t pairs = 0, misses = 0, bad = 0, mallocs = 0;
+  uint64_t bytes = 0;
+};
+
+__device__ inline void add_counters(uint64_t* ctr, const lane_counts& c) {
+  const uint64_t v[5] = {wave_sum(c.pairs), wave_sum(c.misses), wave_sum(c.bad),
+                         wave_sum(c.bytes), wave_sum(c.mallocs)};
+  if (wave::lane() == 0) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      if (v[i]) utils::atomic_add(&ctr[i], v[i]);
   }
 }
 
-// The wave writes each participating lane's buffer in turn, 16 bytes per
-// lane per step, so every store instruction is one contiguous 1 KiB run.
+// Payload fill and verify, spread over the whole wave: the buffers of the
+// participating lanes are cut into 16-byte chunks numbered buffer after
+// buffer (lane order), and lane r handles chunks r, r + 64, r + 128, ...
+// A chunk's owner is the last lane whose chunk prefix is <= the chunk index,
+// found by a 6-step binary search over the prefixes (one per lane, read with
+// ds_bpermute).  Every lane issues its loads back to back, so a verify costs
+// about one memory round trip instead of one per buffer.  Needs all 64 lanes
+// of the wave active.
+struct chunk_plan {
+  uint32_t prefix;  // chunks owned by lower lanes
+  uint32_t total;   // chunks of the whole wave
+};
+
+__device__ inline chunk_plan plan_chunks(bool on, uint64_t words) {
+  const uint32_t n = on ? (uint32_t)((words + 1) / 2) : 0u;
+  chunk_plan c;
+  wave::group_scan(n, ~0ULL, c.prefix, c.total);
+  return c;
+}
+
+__device__ __forceinline__ int chunk_owner(uint32_t prefix, uint32_t c) {
+  int l = 0;
+  for (int step = 32; step; step >>= 1) {
+    const uint32_t p = wave::fetch(prefix, l + step);
+    if (p <= c) l += step;
+  }
+  return l;
+}
+
+// The chunk loops are wave-uniform: every lane runs every pass and takes
+// part in the ds_bpermute reads (a disabled source lane would read as 0);
+// only the memory access itself is predicated on the chunk existing.
 __device__ inline void wave_fill(bool on, uint64_t* p, uint64_t words, uint64_t pat) {
-  const uint64_t act = wave::active();
-  const uint32_t k = wave::count(act);
-  const uint32_t rank = wave::count(act & wave::lanes_below());
-  for (uint64_t m = __ballot(on); m; m &= m - 1) {
-    const int l = wave::first(m);
-    uint64_t* q = reinterpret_cast<uint64_t*>(wave::readlane((uint64_t)p, l));
-    const uint64_t n = wave::readlane(words, l);
-    const uint64_t v = wave::readlane(pat, l);
-    for (uint64_t i = 2ULL * rank; i < n; i += 2ULL * k) {
+  const chunk_plan cp = plan_chunks(on, words);
+  const uint32_t lane = wave::lane();
+  for (uint32_t base = 0; base < cp.total; base += 64) {
+    const uint32_t c = base + lane;
+    const bool in = c < cp.total;
+    const int l = chunk_owner(cp.prefix, in ? c : cp.total - 1);
+    uint64_t* q = reinterpret_cast<uint64_t*>(wave::fetch((uint64_t)p, l));
+    const uint64_t n = wave::fetch(words, l);
+    const uint64_t v = wave::fetch(pat, l);
+    const uint64_t i = 2ULL * ((uint64_t)c - wave::fetch(cp.prefix, l));
+    if (in) {
       if (i + 1 < n)
         *reinterpret_cast<ulonglong2*>(q + i) = make_ulonglong2(v, v);
       else
@@ -52,19 +94,21 @@ __device__ inline void wave_fill(bool on, uint64_t* p, uint64_t words, uint64_t 
   }
 }
 
-// Cooperative check of the same layout; returns this lane's mismatch count.
-__device__ inline uint64_t wave_verify(bool on, const uint64_t* p, uint64_t words,
+// Returns this lane's share of the mismatching words.
+__device__ inline uint32_t wave_verify(bool on, const uint64_t* p, uint64_t words,
                                        uint64_t pat) {
-  const uint64_t act = wave::active();
-  const uint32_t k = wave::count(act);
-  const uint32_t rank = wave::count(act & wave::lanes_below());
-  uint64_t bad = 0;
-  for (uint64_t m = __ballot(on); m; m &= m - 1) {
-    const int l = wave::first(m);
-    const uint64_t* q = reinterpret_cast<const uint64_t*>(wave::readlane((uint64_t)p, l));
-    const uint64_t n = wave::readlane(words, l);
-    const uint64_t v = wave::readlane(pat, l);
-    for (uint64_t i = 2ULL * rank; i < n; i += 2ULL * k) {
+  const chunk_plan cp = plan_chunks(on, words);
+  const uint32_t lane = wave::lane();
+  uint32_t bad = 0;
+  for (uint32_t base = 0; base < cp.total; base += 64) {
+    const uint32_t c = base + lane;
+    const bool in = c < cp.total;
+    const int l = chunk_owner(cp.prefix, in ? c : cp.total - 1);
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(wave::fetch((uint64_t)p, l));
+    const uint64_t n = wave::fetch(words, l);
+    const uint64_t v = wave::fetch(pat, l);
+    const uint64_t i = 2ULL * ((uint64_t)c - wave::fetch(cp.prefix, l));
+    if (in) {
       if (i + 1 < n) {
         const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(q + i);
         bad += (x.x != v) + (x.y != v);
@@ -119,21 +163,21 @@ __global__ void k_wave_stream(gal_state* s, const gal_wave_step* steps, uint64_t
 // Plain method: malloc(size), write the thread id, read it back, free.
 __global__ void k_plain(gal_state* s, uint64_t n, uint64_t size, uint64_t* ctr) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t c[5] = {0, 0, 0, 0, 0};
+  lane_counts c;
   if (tid < n) {
     volatile uint32_t* p = static_cast<volatile uint32_t*>(detail::malloc_impl(s, size));
-    c[GAL_CTR_MALLOCS] = 1;
+    c.mallocs = 1;
     if (!p) {
-      c[GAL_CTR_MISSES] = 1;
+      c.misses = 1;
     } else {
       p[0] = (uint32_t)tid;
-      c[GAL_CTR_BAD] = p[0] != (uint32_t)tid;
-      c[GAL_CTR_BYTES] = 8;
+      c.bad = p[0] != (uint32_t)tid;
+      c.bytes = 8;
       detail::free_impl(s, (void*)p);
-      c[GAL_CTR_PAIRS] = 1;
+      c.pairs = 1;
     }
   }
-  add_counters(ctr, c, 5);
+  add_counters(ctr, c);
 }
 
 // Mixed-size churn: one live allocation per thread, replaced in round 0, in
@@ -143,38 +187,38 @@ __global__ void k_churn(gal_state* s, uint64_t seed, uint64_t n, uint32_t rounds
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = tid < n;
   uint64_t* live = nullptr;
-  uint64_t live_words = 0;
+  uint32_t live_words = 0;
   uint32_t live_round = 0;
-  uint64_t c[5] = {0, 0, 0, 0, 0};
+  lane_counts c;
   for (uint32_t r = 0; r <= rounds; ++r) {
     const bool replace =
         active && (r == 0 || r == rounds || gal_churn_coin(seed, tid, r) != 0);
     const bool do_free = replace && live != nullptr;
-    c[GAL_CTR_BAD] += wave_verify(do_free, live, live_words, gal_churn_pattern(tid, live_round));
+    c.bad += wave_verify(do_free, live, live_words, gal_churn_pattern(tid, live_round));
     if (do_free) {
       detail::free_impl(s, live);
-      c[GAL_CTR_PAIRS] += 1;
-      c[GAL_CTR_BYTES] += live_words * 8;
+      c.pairs += 1;
+      c.bytes += (uint64_t)live_words * 8;
       live = nullptr;
     }
     if (r == rounds) break;
-    const uint64_t size = gal_churn_size(seed, tid, r);
+    const uint32_t size = (uint32_t)gal_churn_size(seed, tid, r);
     uint64_t* np = nullptr;
     if (replace) {
       np = static_cast<uint64_t*>(detail::malloc_impl(s, size));
-      c[GAL_CTR_MALLOCS] += 1;
-      if (!np) c[GAL_CTR_MISSES] += 1;
+      c.mallocs += 1;
+      if (!np) c.misses += 1;
     }
     const bool filled = replace && np != nullptr;
     if (filled) {
       live = np;
-      live_words = gal_fill_words(size);
+      live_words = (uint32_t)gal_fill_words(size);
       live_round = r;
-      c[GAL_CTR_BYTES] += live_words * 8;
+      c.bytes += (uint64_t)live_words * 8;
     }
     wave_fill(filled, np, gal_fill_words(size), gal_churn_pattern(tid, r));
   }
-  add_counters(ctr, c, 5);
+  add_counters(ctr, c);
 }
 
 // Atomic-rate probe: op i adds 1 to word ((i / lanes_per_address) * stride)

@@ -76,6 +76,10 @@ __device__ __forceinline__ uint64_t bytes_to_offset(const gal_state* s,
          (seg << (s->seg_shift - s->smallest_bits));
 }
 
+__device__ __forceinline__ gal_block* block_at(const gal_state* s, uint32_t blk) {
+  return s->blocks + block_position(blk, s->block_bits);
+}
+
 __device__ __forceinline__ void add_stat(gal_state* s, int idx, uint64_t v) {
   atomic_add(&s->stats[idx], (unsigned long long)v);
 }
@@ -84,7 +88,7 @@ __device__ __forceinline__ void add_stat(gal_state* s, int idx, uint64_t v) {
 
 // Segment ring return, gallatin.cuh:1637-1726 return_block with
 // memory_table.cuh:1158-1241 reserve_segment_slot / finish_freeing_block.
-__device__ inline void return_block(gal_state* s, uint32_t blk, uint64_t seg,
+__device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t seg,
                                     uint32_t t) {
   const uint64_t nb = nb_of(s, t);
   gal_seg_queue* q = &s->seg_queue[seg];
@@ -122,7 +126,7 @@ __device__ inline void return_block(gal_state* s, uint32_t blk, uint64_t seg,
 }
 
 // gallatin.cuh:1732-1766 free_block.
-__device__ inline void free_block(gal_state* s, uint32_t blk) {
+__device__ __noinline__ void free_block(gal_state* s, uint32_t blk) {
   const uint64_t seg = (uint64_t)blk >> s->bps_shift;
   const uint32_t t = (uint32_t)(atomic_load(&s->seg_state[seg]) & 0xFFFF);
   if (t >= s->num_trees) {
@@ -134,11 +138,11 @@ __device__ inline void free_block(gal_state* s, uint32_t blk) {
 
 // block.cuh:87-98 block_free_multiple + 200-212 reset_free as used by
 // gallatin.cuh:1774-1827 free_offset.
-__device__ inline void block_release(gal_state* s, uint32_t blk, uint32_t k) {
-  const uint32_t old = atomic_add(&s->blocks[blk].free_counter, k);
+__device__ __forceinline__ void block_release(gal_state* s, uint32_t blk, uint32_t k) {
+  const uint32_t old = atomic_add(&block_at(s, blk)->free_counter, k);
   if (old + k == kSlicesPerBlock) {
     utils::fence_acquire();
-    atomic_exch(&s->blocks[blk].free_counter, 0u);
+    atomic_exch(&block_at(s, blk)->free_counter, 0u);
     free_block(s, blk);
   }
 }
@@ -168,7 +172,7 @@ __device__ inline uint32_t get_block(gal_state* s, uint64_t seg, uint32_t t,
     add_stat(s, kStatErrors, 1);
     return kNullBlock;
   }
-  atomic_exch(&s->blocks[blk].malloc_counter, t << kTreeShift);  // init_malloc
+  atomic_exch(&block_at(s, blk)->malloc_counter, t << kTreeShift);  // init_malloc
   last = (ac == 0);
   return blk;
 }
@@ -176,7 +180,7 @@ __device__ inline uint32_t get_block(gal_state* s, uint64_t seg, uint32_t t,
 // gallatin.cuh:1283-1302 submit_segment_for_free: the chunk id is reset
 // before the run goes back to the segment tree, so a concurrent gather can
 // never find the head still marked.
-__device__ inline void free_segments(gal_state* s, uint64_t seg, uint64_t n) {
+__device__ __noinline__ void free_segments(gal_state* s, uint64_t seg, uint64_t n) {
   atomic_or(&s->seg_state[seg], (uint64_t)kFreeTree);
   utils::fence_release();
   veb::return_multiple(s->seg_tree, seg, n);
@@ -331,7 +335,7 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, bool det)
 }
 
 // gallatin.cuh:1083-1138 replace_block.
-__device__ inline void replace_block(gal_state* s, uint32_t t, uint32_t slot,
+__device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t slot,
                                      uint32_t blk, bool det) {
   uint32_t* sp = s->slots + s->slot_off[t] + slot;
   uint32_t seen = 0;
@@ -352,7 +356,7 @@ __device__ inline void replace_block(gal_state* s, uint32_t t, uint32_t slot,
 
 // Slice phase of malloc: gallatin.cuh:865-1078 malloc_slice_allocation for
 // every size class present in the wave at once.
-__device__ inline void malloc_slices(gal_state* s, bool want, uint32_t t,
+__device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t t,
                                      uint32_t cnt, uint64_t& out_off) {
   const bool det = deterministic(s);
   const uint32_t lane = wave::lane();
@@ -379,7 +383,7 @@ __device__ inline void malloc_slices(gal_state* s, bool want, uint32_t t,
       wave::group_scan(pending ? cnt : 0u, grp, prefix, total);
     }
     uint32_t old = 0;
-    if (have && is_leader) old = atomic_add(&s->blocks[blk].malloc_counter, total);
+    if (have && is_leader) old = atomic_add(&block_at(s, blk)->malloc_counter, total);
     old = wave::fetch(old, leader);
     const uint32_t al = (old & kCountMask) + prefix;
     const bool start_valid = al <= 4095u;
@@ -394,14 +398,14 @@ __device__ inline void malloc_slices(gal_state* s, bool want, uint32_t t,
       wave::group_scan(leftover, have ? grp : 0ULL, lp, lsum);
       bool completes = false;
       if (have && is_leader && lsum) {
-        const uint32_t of = atomic_add(&s->blocks[blk].free_counter, lsum);
+        const uint32_t of = atomic_add(&block_at(s, blk)->free_counter, lsum);
         completes = of + lsum == kSlicesPerBlock;
       }
       for (uint64_t m = __ballot(completes); m; m &= m - 1) {
         const uint32_t b = wave::readlane(blk, wave::first(m));
         if (wave::is_first()) {
           utils::fence_acquire();
-          atomic_exch(&s->blocks[b].free_counter, 0u);
+          atomic_exch(&block_at(s, b)->free_counter, 0u);
           free_block(s, b);
         }
       }
@@ -430,18 +434,18 @@ __device__ inline void malloc_slices(gal_state* s, bool want, uint32_t t,
 
 // gallatin.cuh:828-861 malloc_block_allocation + block.cuh:288-340
 // malloc_fill_block, retried as gallatin.cuh:1226-1231.  Byte offset.
-__device__ inline uint64_t malloc_block_path(gal_state* s, uint32_t bt, bool det) {
+__device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, bool det) {
   for (uint32_t attempt = 0; attempt < s->lim_block; ++attempt) {
     const uint32_t blk = request_new_block(s, bt, det);
     if (blk == kNullBlock) continue;
     uint32_t res = 0;
     if (wave::is_first()) {
-      const uint32_t merged = atomic_add(&s->blocks[blk].malloc_counter, 4096u);
+      const uint32_t merged = atomic_add(&block_at(s, blk)->malloc_counter, 4096u);
       const uint32_t oldc = merged & kCountMask;
       if ((merged >> kTreeShift) != bt) {  // check_valid -> trap
         add_stat(s, kStatErrors, 1);
       } else {
-        atomic_add(&s->blocks[blk].free_counter, 4095u - oldc);
+        atomic_add(&block_at(s, blk)->free_counter, 4095u - oldc);
         if (oldc != 0)
           block_release(s, blk, 1);
         else
@@ -456,7 +460,7 @@ __device__ inline uint64_t malloc_block_path(gal_state* s, uint32_t bt, bool det
 // gallatin.cuh:790-826 malloc_segment_allocation under the segment lock,
 // retried as gallatin.cuh:1246-1251.  Byte offset.  The lock is taken by one
 // lane of one request at a time, so lanes of a wave never wait on each other.
-__device__ inline uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
+__device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
   const uint32_t T = s->num_trees;
   if (n == 0 || n > s->num_segments || n + T + 1 >= kFreeTree) return kFail;
   for (uint32_t attempt = 0; attempt < s->lim_segment; ++attempt) {
@@ -494,7 +498,7 @@ __device__ inline uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
 }
 
 // gallatin.cuh:1186-1276 malloc.
-__device__ inline void* malloc_impl(gal_state* s, uint64_t size) {
+__device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
   const bool det = deterministic(s);
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
@@ -539,7 +543,7 @@ __device__ inline void* malloc_impl(gal_state* s, uint64_t size) {
 }
 
 // gallatin.cuh:1378-1461 free + 1774-1827 free_offset.
-__device__ inline void free_impl(gal_state* s, void* ptr) {
+__device__ __forceinline__ void free_impl(gal_state* s, void* ptr) {
   const bool det = deterministic(s);
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
@@ -589,14 +593,14 @@ __device__ inline void free_impl(gal_state* s, void* ptr) {
   bool completes = false;
   if (is_leader) {
     const uint32_t k = wave::count(grp);
-    const uint32_t old = atomic_add(&s->blocks[blk].free_counter, k);
+    const uint32_t old = atomic_add(&block_at(s, blk)->free_counter, k);
     completes = old + k == kSlicesPerBlock;
   }
   for (uint64_t m = __ballot(completes); m; m &= m - 1) {
     const uint32_t b = wave::readlane(blk, wave::first(m));
     if (wave::is_first()) {
       utils::fence_acquire();
-      atomic_exch(&s->blocks[b].free_counter, 0u);
+      atomic_exch(&block_at(s, b)->free_counter, 0u);
       free_block(s, b);
     }
   }

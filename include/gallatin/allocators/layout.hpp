@@ -54,6 +54,21 @@ struct gal_block {
   uint32_t free_counter;    // freed + pre-credited slices
 };
 
+// Position of block b's counters in the block array of 2^k entries.  The
+// blocks that are hot at the same time (pinned blocks, numbered segment by
+// segment) would otherwise share a few 4 KiB regions, and device-scope
+// atomics on one region serialise at one memory-side unit (~90 M/s); this
+// bijection of [0, 2^k) (xorshift, odd multiply, xorshift) scatters
+// neighbouring ids over the whole array.
+__host__ __device__ inline uint64_t block_position(uint64_t b, uint32_t k) {
+  if (k < 10) return b;
+  const uint32_t h = (k + 1) / 2;
+  const uint64_t mask = (1ULL << k) - 1;
+  uint64_t x = b ^ (b >> h);
+  x = (x * 0x9E3779B97F4A7C15ULL) & mask;
+  return x ^ (x >> h);
+}
+
 struct gal_tree {
   uint64_t* words;                     // layer 0 (top) .. num_layers-1 (bottom)
   uint64_t universe;                   // bits in the bottom layer
@@ -95,7 +110,7 @@ struct gal_state {
   uint32_t block_tree_bits;      // log2(smallest * 4096)
   uint32_t flags;
   uint32_t total_slots;
-  uint32_t pad0;
+  uint32_t block_bits;           // block array holds 2^block_bits entries
   // retry limits (defaults kRequestBlockAttempts, kSliceAttempts, ...)
   uint32_t lim_request;
   uint32_t lim_slice;
@@ -106,7 +121,7 @@ struct gal_state {
   uint64_t* seg_state;           // [N]
   gal_seg_queue* seg_queue;      // [N]
   uint32_t* ring;                // [N * B] recycled block ids
-  gal_block* blocks;             // [N * B]
+  gal_block* blocks;             // [2^block_bits], at block_position()
   uint32_t* slots;               // pinned blocks of all size classes
   uint32_t* locks;               // [1] tree lock bits (gallatin.cuh:314)
   unsigned long long* stats;     // [kStatCount]
