@@ -41,7 +41,8 @@ def parse():
     p.add_argument("--pool-gib", type=int, default=8)
     p.add_argument("--seed", type=int, default=1234)
     p.add_argument("--slots0", type=int, default=128)
-    p.add_argument("--slots-min", type=int, default=4)
+    p.add_argument("--slots-min", type=int, default=32)
+    p.add_argument("--partitions", type=int, default=0)
     p.add_argument("--flags", type=int, default=0)
     p.add_argument("--block-threads", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -138,7 +139,7 @@ def main():
     torch.cuda.set_device(0)
     alloc = G.Gallatin(16 << 20, 16, 4096).generate_on_device(
         args.pool_gib << 30, args.seed, print_info=False, flags=args.flags,
-        slots0=args.slots0, slots_min=args.slots_min)
+        slots0=args.slots0, slots_min=args.slots_min, partitions=args.partitions)
     ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
     for w in range(args.warmup):
         alloc.run_churn(args.seed + w, args.threads, args.rounds, counters=ctr,
@@ -157,6 +158,7 @@ def main():
     step_ms = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
     c = ctr.cpu().tolist()
     info = alloc.info()
+    partitions = alloc.partitions
     pairs, misses, bad, nbytes = c[G.CTR_PAIRS], c[G.CTR_MISSES], c[G.CTR_BAD], c[G.CTR_BYTES]
     value = pairs / (total_ms / 1e3)
     avg_ms = total_ms / args.steps
@@ -184,6 +186,7 @@ def main():
             "rounds": args.rounds,
             "slots0": args.slots0,
             "slots_min": args.slots_min,
+            "xcd_partitions": partitions,
             "block_threads": args.block_threads,
         },
         "p50_step_ms": step_ms[len(step_ms) // 2],

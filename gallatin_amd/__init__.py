@@ -55,14 +55,15 @@ class Config(ctypes.Structure):
                 ("flags", _u32), ("slots0", _u32), ("slots_min", _u32),
                 ("print_info", _u32), ("request_attempts", _u32),
                 ("slice_attempts", _u32), ("block_attempts", _u32),
-                ("segment_attempts", _u32)]
+                ("segment_attempts", _u32), ("partitions", _u32), ("pad", _u32)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("num_segments", _u64), ("bytes_per_segment", _u64),
                 ("smallest", _u64), ("biggest", _u64),
                 ("blocks_per_segment", _u64), ("num_trees", _u32),
-                ("total_slots", _u32), ("segments_free", _u64),
+                ("total_slots", _u32), ("partitions", _u32), ("pad", _u32),
+                ("segments_free", _u64),
                 ("segments_formatted", _u64), ("segments_multi", _u64),
                 ("tree_owned", _u64 * 24), ("errors", _u64),
                 ("overhead_bytes", _u64), ("pool_base", _u64)]
@@ -94,7 +95,7 @@ def lib():
         L.gal_reboot.argtypes = [_p, _p]
         L.gal_snapshot.argtypes = [_p, _p, _p, _p]
         L.gal_tree_words.restype = ctypes.c_int64
-        L.gal_tree_words.argtypes = [_p, ctypes.c_int, _p, _u64, _p, _p]
+        L.gal_tree_words.argtypes = [_p, ctypes.c_int, ctypes.c_int, _p, _u64, _p, _p]
         L.gal_init_global.argtypes = [_u64, _u64, ctypes.c_int]
         L.gal_global_handle.restype = _p
         L.gal_malloc_bulk.argtypes = [_p, _p, _u64, _p, _u32, _p]
@@ -139,9 +140,10 @@ class Gallatin:
     # ---- host lifecycle (gallatin.cuh:324-421, 643-668, 1855) ----
     def generate_on_device(self, max_bytes, seed, print_info=True,
                            running_calloc=False, *, flags=0, slots0=128,
-                           slots_min=4, limits=None):
+                           slots_min=4, limits=None, partitions=0):
         """limits: (request, slice, block, segment) retry limits; None or 0
-        entries keep the reference's 1000, 5000, 500, 500."""
+        entries keep the reference's 1000, 5000, 500, 500.  partitions: XCD
+        partitions (0 = one per XCD; deterministic mode uses 1)."""
         if running_calloc:
             raise GallatinError("calloc mode is not supported")
         cfg = Config()
@@ -156,6 +158,7 @@ class Gallatin:
         if limits:
             (cfg.request_attempts, cfg.slice_attempts, cfg.block_attempts,
              cfg.segment_attempts) = [int(x or 0) for x in limits]
+        cfg.partitions = partitions
         h = _p()
         _check(lib().gal_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
@@ -169,6 +172,7 @@ class Gallatin:
         self.num_trees = i["num_trees"]
         self.blocks_per_segment = i["blocks_per_segment"]
         self.total_slots = i["total_slots"]
+        self.partitions = i["partitions"]
         self.pool_base = i["pool_base"]
 
     def free_on_device(self):
@@ -215,14 +219,15 @@ class Gallatin:
                                   bl.ctypes.data))
         return st, sl, bl.reshape(-1, 2)
 
-    def tree_layers(self, tree):
-        """Words of each layer (top first); tree -1 = segment tree."""
+    def tree_layers(self, tree, partition=0):
+        """Words of each layer (top first); tree -1 = segment tree, else the
+        class tree of XCD partition `partition`."""
         nl = _u32()
         lw = (_u64 * 4)()
-        n = _check(lib().gal_tree_words(self.handle, tree, None, 0,
+        n = _check(lib().gal_tree_words(self.handle, tree, partition, None, 0,
                                         ctypes.byref(nl), lw))
         out = np.zeros(n, np.uint64)
-        _check(lib().gal_tree_words(self.handle, tree, out.ctypes.data, n,
+        _check(lib().gal_tree_words(self.handle, tree, partition, out.ctypes.data, n,
                                     ctypes.byref(nl), lw))
         layers, off = [], 0
         for layer in range(nl.value):

@@ -76,11 +76,14 @@ __global__ void k_fill_tree(gal_tree t) {
 }
 
 // boot_shared_block_container_one_thread order (gallatin.cuh:195-221), run
-// by one wave so tree searches are cooperative.
+// by one wave so tree searches are cooperative.  Slot i of a class belongs
+// to partition i / (slots / partitions) and takes its block from a segment
+// homed there (the kernel boundary writes every L2 back before use).
 __global__ void k_boot(gal_state* s) {
   for (uint32_t t = 0; t < s->num_trees; ++t) {
+    const uint32_t per = s->slot_count[t] / s->num_parts;
     for (uint32_t i = 0; i < s->slot_count[t]; ++i) {
-      const uint32_t blk = detail::request_new_block(s, t, true);
+      const uint32_t blk = detail::request_new_block(s, t, i / per, true);
       if (blk == kNullBlock) {
         if (wave::is_first()) utils::atomic_store(&s->stats[kStatBootFailed], 1ULL);
         return;
@@ -92,7 +95,8 @@ __global__ void k_boot(gal_state* s) {
 
 // Quiescent drain: pinned blocks give back their unclaimed slices, then
 // formatted segments with every block free are released as return_block
-// would release them.
+// would release them.  Runs after a kernel boundary, so every L2 has been
+// written back and partitions play no part.
 __global__ void k_drain(gal_state* s) {
   for (uint32_t t = 0; t < s->num_trees; ++t) {
     for (uint32_t i = 0; i < s->slot_count[t]; ++i) {
@@ -101,7 +105,9 @@ __global__ void k_drain(gal_state* s) {
         if (blk != kNullBlock) {
           uint32_t c = utils::atomic_load(&detail::block_at(s, blk)->malloc_counter) & kCountMask;
           if (c > 4096u) c = 4096u;
-          if (c < 4096u) detail::block_release(s, blk, 4096u - c);
+          // act as the block's home so a completed segment is released
+          const uint64_t st = utils::atomic_load(&s->seg_state[blk >> s->bps_shift]);
+          if (c < 4096u) detail::block_release(s, blk, 4096u - c, seg_home(st));
         }
       }
     }
@@ -127,10 +133,10 @@ __global__ void k_drain(gal_state* s) {
       const uint64_t sv = wave::readlane(st, l);
       if (wave::is_first()) {
         const uint32_t t = (uint32_t)(sv & 0xFFFF);
-        const uint64_t closed = (0xFFFFFFFFULL << 32) | t;
+        const uint64_t closed = (0xFFFFFFFFULL << 32) | (sv & kSegIdMask);
         if (utils::atomic_cas(&s->seg_state[sg], sv, closed) == sv) {
-          veb::remove(s->sub_trees[t], sg);
-          utils::atomic_or(&s->seg_state[sg], (uint64_t)kFreeTree);
+          veb::remove(detail::class_tree(s, t, seg_home(sv)), sg);
+          detail::clear_chunk_id(s, sg);
           veb::insert(s->seg_tree, sg);
           detail::add_stat(s, kStatSegmentsReleased, 1);
         }
@@ -232,22 +238,41 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   h.lim_slice = cfg->slice_attempts ? cfg->slice_attempts : (uint32_t)kSliceAttempts;
   h.lim_block = cfg->block_attempts ? cfg->block_attempts : (uint32_t)kBlockAttempts;
   h.lim_segment = cfg->segment_attempts ? cfg->segment_attempts : (uint32_t)kSegmentAttempts;
-  // pinned slot counts, shared_block_storage.cuh:251-274
+  // XCD partitions: one per XCD (32 CUs each on MI355X) unless deterministic
+  uint32_t P = cfg->partitions;
+  if (cfg->flags & kFlagDeterministic) {
+    P = 1;
+  } else if (P == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, a->device) != hipSuccess)
+      cus = 32;
+    P = (uint32_t)(cus / 32);
+    // each class pins blocks in every partition at boot: keep that to about
+    // a quarter of the pool
+    const uint64_t fit = N / (4ULL * T);
+    if (P > fit) P = (uint32_t)fit;
+  }
+  if (P < 1) P = 1;
+  if (P > (uint32_t)kMaxParts) P = kMaxParts;
+  h.num_parts = P;
+  // pinned slot counts, shared_block_storage.cuh:251-274, each rounded up to
+  // a multiple of the partitions
   uint32_t p = cfg->slots0 ? cfg->slots0 : 1, total = 0;
   const uint32_t pmin = cfg->slots_min ? cfg->slots_min : 1;
   for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t c = (p + P - 1) / P * P;
     h.slot_off[t] = total;
-    h.slot_count[t] = p;
-    total += p;
+    h.slot_count[t] = c;
+    total += c;
     p /= 2;
     if (p < pmin) p = pmin;
   }
   h.slot_off[T] = total;
   h.total_slots = total;
-  // trees: segment tree + one per class, all in one word arena
+  // trees: segment tree + one per class and partition, in one word arena
   uint64_t words = 0;
   size_tree(h.seg_tree, N, words);
-  for (uint32_t t = 0; t < T; ++t) size_tree(h.sub_trees[t], N, words);
+  for (uint32_t t = 0; t < T * P; ++t) size_tree(h.sub_trees[t], N, words);
   a->tree_words = words;
   // metadata arena
   uint64_t off = 0;
@@ -263,7 +288,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   h.block_bits = (N * B) <= 1 ? 0 : 64u - (uint32_t)__builtin_clzll(N * B - 1);
   const uint64_t o_blocks = take((1ULL << h.block_bits) * sizeof(gal_block));
   const uint64_t o_slots = take((uint64_t)total * sizeof(uint32_t));
-  const uint64_t o_locks = take(256);
+  const uint64_t o_locks = take((1 + (uint64_t)T * P) * sizeof(uint32_t));
   const uint64_t o_stats = take(kStatCount * sizeof(unsigned long long));
   const uint64_t o_trees = take(words * sizeof(uint64_t));
   a->meta_bytes = off;
@@ -291,7 +316,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   uint64_t* tree_base = reinterpret_cast<uint64_t*>(base + o_trees);
   a->tree_base = tree_base;
   h.seg_tree.words = tree_base + reinterpret_cast<uint64_t>(h.seg_tree.words);
-  for (uint32_t t = 0; t < T; ++t)
+  for (uint32_t t = 0; t < T * P; ++t)
     h.sub_trees[t].words = tree_base + reinterpret_cast<uint64_t>(h.sub_trees[t].words);
   a->dev = reinterpret_cast<gal_state*>(base + o_state);
 
@@ -401,12 +426,14 @@ int gal_snapshot(gal_allocator* a, uint64_t* seg_state, uint32_t* slots, uint32_
   return GAL_OK;
 }
 
-int64_t gal_tree_words(gal_allocator* a, int tree, uint64_t* out, uint64_t cap,
-                       uint32_t* num_layers, uint64_t* layer_words) {
+int64_t gal_tree_words(gal_allocator* a, int tree, int partition, uint64_t* out,
+                       uint64_t cap, uint32_t* num_layers, uint64_t* layer_words) {
   if (!a) return fail(GAL_ERR_INVALID, "null allocator");
   const gal_state& h = a->host;
   if (tree < -1 || tree >= (int)h.num_trees) return fail(GAL_ERR_INVALID, "bad tree index");
-  const gal_tree& t = tree < 0 ? h.seg_tree : h.sub_trees[tree];
+  if (partition < 0 || partition >= (int)h.num_parts)
+    return fail(GAL_ERR_INVALID, "bad partition index");
+  const gal_tree& t = tree < 0 ? h.seg_tree : h.sub_trees[tree * h.num_parts + partition];
   const uint64_t n = tree_total_words(t);
   if (num_layers) *num_layers = t.num_layers;
   if (layer_words)
@@ -430,6 +457,7 @@ int gal_get_info(gal_allocator* a, gal_info* out) {
   out->blocks_per_segment = h.blocks_per_segment;
   out->num_trees = h.num_trees;
   out->total_slots = h.total_slots;
+  out->partitions = h.num_parts;
   out->pool_base = (uint64_t)h.memory;
   std::vector<uint64_t> words(a->tree_words);
   hipError_t e = hipDeviceSynchronize();
@@ -443,7 +471,9 @@ int gal_get_info(gal_allocator* a, gal_info* out) {
     return c;
   };
   out->segments_free = fill(h.seg_tree);
-  for (uint32_t t = 0; t < h.num_trees; ++t) out->tree_owned[t] = fill(h.sub_trees[t]);
+  for (uint32_t t = 0; t < h.num_trees; ++t)
+    for (uint32_t p = 0; p < h.num_parts; ++p)
+      out->tree_owned[t] += fill(h.sub_trees[t * h.num_parts + p]);
   std::vector<uint64_t> st(h.num_segments);
   if ((e = hipMemcpy(st.data(), h.seg_state, h.num_segments * 8, hipMemcpyDeviceToHost)) != hipSuccess)
     return hip_fail(e, "info copy");

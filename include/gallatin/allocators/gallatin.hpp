@@ -6,7 +6,7 @@
 // `biggest` in powers of two, one bitmap tree per class plus the segment
 // tree, 4096-slice blocks claimed by one counter atomic, pinned blocks per
 // class, recycled blocks in a per-segment ring, segments returned when all
-// their blocks are back.  The execution is wave64-native:
+// their blocks are back.  The execution is wave64- and XCD-native:
 //
 //  * every entry point is called by the active lanes of a wave together; the
 //    lanes are partitioned by size class (malloc) or block (free) with
@@ -15,8 +15,15 @@
 //    release, whole-block and multi-segment requests) run one request at a
 //    time in lane order with the whole wave available for tree searches;
 //  * tree searches scan 64 words per pass across the wave (veb.hpp);
-//  * metadata reads are agent-scope atomics, payload is released on free and
-//    acquired on malloc, so memory moves safely between XCDs.
+//  * the pinned slots and class trees are split into partitions, one per
+//    XCD.  A formatted segment has a home partition; only waves of that XCD
+//    pin its blocks, so its memory is written through one L2.  A free on the
+//    home XCD therefore only has to wait for its own stores to reach that L2;
+//    a free from another XCD (or of a multi-segment run) writes the L2 back
+//    (agent-scope release).  A segment changes home only when a wave of its
+//    home XCD releases it, after writing its L2 back.
+//  * With one partition (deterministic mode, or partitions = 1) every free
+//    is an agent-scope release and every malloc an agent-scope acquire.
 //
 // Device API (same names as the reference): Gallatin<S,m,M>::malloc / free /
 // get_allocated_size / owns_allocation; host API: generate_on_device /
@@ -53,6 +60,16 @@ __device__ __forceinline__ bool deterministic(const gal_state* s) {
   return (s->flags & kFlagDeterministic) != 0;
 }
 
+// Partition of the calling wave: its XCD (__smid() = xcc << 6 | se << 4 | cu).
+__device__ __forceinline__ uint32_t wave_part(const gal_state* s) {
+  return s->num_parts == 1 ? 0u : (__smid() >> 6) % s->num_parts;
+}
+
+__device__ __forceinline__ const gal_tree& class_tree(const gal_state* s, uint32_t t,
+                                                      uint32_t p) {
+  return s->sub_trees[t * s->num_parts + p];
+}
+
 // blocks of class t per segment (memory_table.cuh:1057-1061)
 __device__ __forceinline__ uint64_t nb_of(const gal_state* s, uint32_t t) {
   return s->blocks_per_segment >> t;
@@ -84,18 +101,37 @@ __device__ __forceinline__ void add_stat(gal_state* s, int idx, uint64_t v) {
   atomic_add(&s->stats[idx], (unsigned long long)v);
 }
 
+// Stores of this wave have reached the L2 of its XCD.
+__device__ __forceinline__ void release_to_xcd() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
+// Chunk id field of seg_state: tree 0xFFFF, home 0 (free segment).
+__device__ inline void clear_chunk_id(gal_state* s, uint64_t seg) {
+  uint64_t cur = atomic_load(&s->seg_state[seg]);
+  for (int g = 0; g < 4096; ++g) {
+    const uint64_t want = (cur & ~kSegIdMask) | kFreeTree;
+    const uint64_t seen = atomic_cas(&s->seg_state[seg], cur, want);
+    if (seen == cur) return;
+    cur = seen;
+  }
+  add_stat(s, kStatErrors, 1);
+}
+
 // ----------------------------------------------------------- per lane ops
 
 // Segment ring return, gallatin.cuh:1637-1726 return_block with
 // memory_table.cuh:1158-1241 reserve_segment_slot / finish_freeing_block.
+// `h` is the segment's home partition, `my_p` the caller's.
 __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t seg,
-                                    uint32_t t) {
+                                          uint32_t t, uint32_t h, uint32_t my_p) {
   const uint64_t nb = nb_of(s, t);
   gal_seg_queue* q = &s->seg_queue[seg];
   const uint32_t pos = atomic_add(&q->q_free, 1u);
   const uint32_t r = pos & (uint32_t)(nb - 1);
   atomic_exch(&s->ring[(seg << s->bps_shift) + r], blk);
-  utils::fence_release();
+  release_to_xcd();
   // finalize_free_queue: tickets complete in order
   for (uint32_t spin = 0; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
     if (spin >= kSpinCap) {
@@ -106,18 +142,23 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
   }
   // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) in exact integer form
   if (10ULL * r > nb && r >= 1 && 10ULL * (r - 1) < nb)
-    veb::insert(s->sub_trees[t], seg);
+    veb::insert(class_tree(s, t, h), seg);
   const uint64_t old = atomic_add(&s->seg_state[seg], kActiveOne);
   const int32_t rid = (int32_t)(old >> 32);
-  add_stat(s, kStatBlocksReturned, 1);
   if ((int64_t)rid == (int64_t)nb - 2) {
-    const uint64_t full = ((uint64_t)(uint32_t)(nb - 1) << 32) | t;
-    const uint64_t closed = (0xFFFFFFFFULL << 32) | t;
+    if (my_p != h) {
+      // every block is back but this wave is not on the home XCD, whose L2
+      // may still hold lines of the segment: keep it formatted for its home
+      veb::insert(class_tree(s, t, h), seg);
+      return;
+    }
+    const uint64_t full = ((uint64_t)(uint32_t)(nb - 1) << 32) | (old & kSegIdMask);
+    const uint64_t closed = (0xFFFFFFFFULL << 32) | (old & kSegIdMask);
     if (atomic_cas(&s->seg_state[seg], full, closed) == full) {
       // all blocks back: release the segment (gallatin.cuh:1680-1722)
-      veb::remove(s->sub_trees[t], seg);
+      veb::remove(class_tree(s, t, h), seg);
       utils::fence_release();
-      atomic_or(&s->seg_state[seg], (uint64_t)kFreeTree);
+      clear_chunk_id(s, seg);
       utils::fence_release();
       veb::insert(s->seg_tree, seg);
       add_stat(s, kStatSegmentsReleased, 1);
@@ -126,39 +167,38 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
 }
 
 // gallatin.cuh:1732-1766 free_block.
-__device__ __noinline__ void free_block(gal_state* s, uint32_t blk) {
+__device__ __noinline__ void free_block(gal_state* s, uint32_t blk, uint32_t my_p) {
   const uint64_t seg = (uint64_t)blk >> s->bps_shift;
-  const uint32_t t = (uint32_t)(atomic_load(&s->seg_state[seg]) & 0xFFFF);
+  const uint64_t st = atomic_load(&s->seg_state[seg]);
+  const uint32_t t = (uint32_t)(st & 0xFFFF);
   if (t >= s->num_trees) {
     add_stat(s, kStatErrors, 1);
     return;
   }
-  return_block(s, blk, seg, t);
+  return_block(s, blk, seg, t, seg_home(st), my_p);
 }
 
 // block.cuh:87-98 block_free_multiple + 200-212 reset_free as used by
 // gallatin.cuh:1774-1827 free_offset.
-__device__ __forceinline__ void block_release(gal_state* s, uint32_t blk, uint32_t k) {
+__device__ __forceinline__ void block_release(gal_state* s, uint32_t blk, uint32_t k,
+                                              uint32_t my_p) {
   const uint32_t old = atomic_add(&block_at(s, blk)->free_counter, k);
   if (old + k == kSlicesPerBlock) {
-    utils::fence_acquire();
     atomic_exch(&block_at(s, blk)->free_counter, 0u);
-    free_block(s, blk);
+    free_block(s, blk, my_p);
   }
 }
 
-// memory_table.cuh:915-993 get_block.  The packed segment word gives the
-// active count and the owning class in one atomic.
-__device__ inline uint32_t get_block(gal_state* s, uint64_t seg, uint32_t t,
-                                     uint32_t& last) {
-  last = 0;
+// memory_table.cuh:915-993 get_block for a segment of class t, home p.  The
+// packed segment word gives the active count and the owner in one atomic.
+// Returns the block id, with bit 32 set when it was the segment's last.
+__device__ inline uint64_t get_block(gal_state* s, uint64_t seg, uint32_t t, uint32_t p) {
   const uint64_t old = atomic_sub(&s->seg_state[seg], kActiveOne);
   const int32_t ac = (int32_t)(old >> 32);
-  if (ac < 0 || (uint32_t)(old & 0xFFFF) != t) {
+  if (ac < 0 || (uint32_t)(old & 0xFFFF) != t || seg_home(old) != p) {
     atomic_add(&s->seg_state[seg], kActiveOne);
     return kNullBlock;
   }
-  utils::fence_acquire();
   const uint64_t nb = nb_of(s, t);
   const uint32_t q = atomic_add(&s->seg_queue[seg].q_malloc, 1u);
   uint32_t blk;
@@ -172,78 +212,71 @@ __device__ inline uint32_t get_block(gal_state* s, uint64_t seg, uint32_t t,
     add_stat(s, kStatErrors, 1);
     return kNullBlock;
   }
-  atomic_exch(&block_at(s, blk)->malloc_counter, t << kTreeShift);  // init_malloc
-  last = (ac == 0);
-  return blk;
+  // init_malloc (block.cuh:215-224): the tag carries class and home
+  atomic_exch(&block_at(s, blk)->malloc_counter, block_tag(t, p) << kTreeShift);
+  return (uint64_t)blk | ((uint64_t)(ac == 0) << 32);
 }
 
 // gallatin.cuh:1283-1302 submit_segment_for_free: the chunk id is reset
 // before the run goes back to the segment tree, so a concurrent gather can
 // never find the head still marked.
 __device__ __noinline__ void free_segments(gal_state* s, uint64_t seg, uint64_t n) {
-  atomic_or(&s->seg_state[seg], (uint64_t)kFreeTree);
+  clear_chunk_id(s, seg);
   utils::fence_release();
   veb::return_multiple(s->seg_tree, seg, n);
 }
 
-// shared_block_storage.cuh:119-134 get_valid_block_index + get_my_block:
-// from `start`, skip empty slots (at most 30 steps); returns the slot's block.
-__device__ inline uint32_t pinned_block(const gal_state* s, uint32_t t,
-                                        uint32_t start, uint32_t& slot) {
-  const uint32_t n = s->slot_count[t];
-  uint32_t* sl = s->slots + s->slot_off[t];
-  const int64_t original = start;
-  int64_t i = start;
+// shared_block_storage.cuh:119-134 get_valid_block_index + get_my_block for
+// the slots of class t in partition p: from the start slot skip empty slots
+// (at most 30 steps).  Returns (slot << 32) | block.
+__device__ inline uint64_t pinned_block(const gal_state* s, uint32_t t, uint32_t p) {
+  const uint32_t per = s->slot_count[t] / s->num_parts;
+  const uint32_t base = p * per;
+  uint32_t rel = 0;
+  if (!deterministic(s)) {
+    const uint32_t mix = ((__smid() & 63u) * 0x9E3779B1u) >> 26;  // 0..63
+    rel = mix % per;
+  }
+  const uint32_t* sl = s->slots + s->slot_off[t] + base;
+  const int64_t original = rel;
+  int64_t i = rel;
   uint32_t blk = atomic_load(&sl[i]);
   int counter = 0;
   while (blk == kNullBlock && i != original - 1) {
-    i = (i + 1) % n;
+    i = (i + 1) % per;
     blk = atomic_load(&sl[i]);
     if (++counter >= kSlotScanCutoff) break;
   }
-  slot = (uint32_t)i;
-  return blk;
-}
-
-// First slot to try for class t.  Deterministic mode: 0 (as smid 0).  Else
-// the slots of a class are split into 8 groups, one per XCD, and the CU id
-// hashes into its XCD's group, so the counters a CU hits stay in the slots
-// its XCD shares (shared_block_storage.cuh:120 uses smid % slots).
-__device__ __forceinline__ uint32_t slot_start(const gal_state* s, uint32_t t) {
-  if (deterministic(s)) return 0;
-  const uint32_t n = s->slot_count[t];
-  const uint32_t id = __smid();
-  const uint32_t mix = ((id & 63u) * 0x9E3779B1u) >> 26;  // 0..63
-  if (n >= 16) {
-    const uint32_t per = n >> 3;
-    return ((id >> 6) & 7u) * per + mix % per;
-  }
-  return (mix + ((id >> 6) & 7u) * 7u) % n;
+  return ((uint64_t)(base + (uint32_t)i) << 32) | blk;
 }
 
 // --------------------------------------------------------- wave ops
 // Functions below are called with identical arguments on every active lane
 // and return wave-uniform results.
 
-__device__ __forceinline__ bool try_lock(gal_state* s, uint32_t bit) {
+__device__ __forceinline__ bool try_lock(gal_state* s, uint32_t idx) {
   uint32_t got = 0;
-  if (wave::is_first()) got = (atomic_or(s->locks, 1u << bit) & (1u << bit)) == 0;
+  if (wave::is_first()) got = atomic_exch(&s->locks[idx], 1u) == 0u;
   return wave::readfirst(got) != 0;
 }
-__device__ __forceinline__ void unlock(gal_state* s, uint32_t bit) {
+__device__ __forceinline__ void unlock(gal_state* s, uint32_t idx) {
   utils::fence_release();
-  if (wave::is_first()) atomic_and(s->locks, ~(1u << bit));
+  if (wave::is_first()) atomic_store(&s->locks[idx], 0u);
+}
+__device__ __forceinline__ uint32_t tree_lock(const gal_state* s, uint32_t t, uint32_t p) {
+  return 1 + t * s->num_parts + p;  // lock 0 guards multi-segment gathers
 }
 
-// memory_table.cuh:746-818 setup_segment: claim the chunk id, clear the ring
-// (whole wave), zero the queue counters, then publish nb-1 free blocks.
-__device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t) {
+// memory_table.cuh:746-818 setup_segment: claim the chunk id for class t,
+// home p, clear the ring (whole wave), zero the queue counters, then publish
+// nb-1 free blocks.
+__device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t, uint32_t p) {
   uint32_t ok = 0;
   if (wave::is_first()) {
     uint64_t cur = atomic_load(&s->seg_state[seg]);
     for (int g = 0; g < 4096; ++g) {
       if ((cur & 0xFFFF) != kFreeTree) break;
-      const uint64_t want = (uint64_t)((cur & ~0xFFFFULL) | t);
+      const uint64_t want = (cur & ~kSegIdMask) | seg_id(t, p);
       const uint64_t seen = atomic_cas(&s->seg_state[seg], cur, want);
       if (seen == cur) {
         ok = 1;
@@ -265,7 +298,7 @@ __device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t) {
     atomic_store(&q->q_free, 0u);
     atomic_store(&q->q_final, 0u);
   }
-  utils::fence_release();
+  release_to_xcd();
   if (wave::is_first()) {
     atomic_add(&s->seg_state[seg], nb << 32);  // active -1 -> nb-1
     add_stat(s, kStatSegmentsSetUp, 1);
@@ -274,22 +307,22 @@ __device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t) {
 }
 
 // gallatin.cuh:1465-1510 gather_new_segment: smallest free segment.
-__device__ inline uint64_t gather_new_segment(gal_state* s, uint32_t t) {
+__device__ inline uint64_t gather_new_segment(gal_state* s, uint32_t t, uint32_t p) {
   const uint64_t seg = veb::take_first(s->seg_tree);
   if (seg == kFail) return kFail;
-  if (!setup_segment(s, seg, t)) {
+  if (!setup_segment(s, seg, t, p)) {
     if (wave::is_first()) add_stat(s, kStatErrors, 1);
     return kFail;
   }
-  if (wave::is_first()) veb::insert(s->sub_trees[t], seg);
+  if (wave::is_first()) veb::insert(class_tree(s, t, p), seg);
   return seg;
 }
 
-// Segment of class t to take a block from: smallest id in deterministic
-// mode (find_first_valid_index, veb.cuh:940-944), else from a hashed start
-// (find_random_valid_index, veb.cuh:946-977) wrapping to 0.
-__device__ inline uint64_t find_segment(gal_state* s, uint32_t t, bool det) {
-  const gal_tree& tr = s->sub_trees[t];
+// Segment of class t (partition p) to take a block from: smallest id in
+// deterministic mode (find_first_valid_index, veb.cuh:940-944), else from a
+// hashed start (find_random_valid_index, veb.cuh:946-977) wrapping to 0.
+__device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bool det) {
+  const gal_tree& tr = class_tree(s, t, p);
   if (det) return veb::find_ge(tr, 0);
   const uint64_t gid = wave::readfirst(
       (uint64_t)threadIdx.x + (uint64_t)blockIdx.x * blockDim.x);
@@ -301,54 +334,54 @@ __device__ inline uint64_t find_segment(gal_state* s, uint32_t t, bool det) {
 }
 
 // gallatin.cuh:1525-1631 request_new_block_from_tree.
-__device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, bool det) {
+__device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t p,
+                                             bool det) {
   int attempts = 0;
+  const uint32_t lk = tree_lock(s, t, p);
   for (uint32_t spins = 0; attempts < (int)s->lim_request && spins < kSpinCap;
        ++spins) {
-    uint64_t seg = find_segment(s, t, det);
+    uint64_t seg = find_segment(s, t, p, det);
     if (seg == kFail) {
-      if (!try_lock(s, t)) {
+      if (!try_lock(s, lk)) {
         utils::backoff();
         continue;
       }
-      seg = gather_new_segment(s, t);
-      unlock(s, t);
+      seg = gather_new_segment(s, t, p);
+      unlock(s, lk);
       if (seg == kFail) {
         ++attempts;
         continue;
       }
     }
-    uint32_t blk = kNullBlock, last = 0;
-    if (wave::is_first()) blk = get_block(s, seg, t, last);
-    blk = wave::readfirst(blk);
-    if (wave::readfirst(last)) {
+    uint64_t got = kNullBlock;
+    if (wave::is_first()) got = get_block(s, seg, t, p);
+    got = wave::readfirst(got);
+    if (got >> 32) {
       // segment fully handed out: detach it and prefetch the next one
-      if (wave::is_first()) veb::remove(s->sub_trees[t], seg);
-      if (try_lock(s, t)) {
-        gather_new_segment(s, t);
-        unlock(s, t);
+      if (wave::is_first()) veb::remove(class_tree(s, t, p), seg);
+      if (try_lock(s, lk)) {
+        gather_new_segment(s, t, p);
+        unlock(s, lk);
       }
     }
+    const uint32_t blk = (uint32_t)got;
     if (blk != kNullBlock) return blk;
   }
   return kNullBlock;
 }
 
 // gallatin.cuh:1083-1138 replace_block.
-__device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t slot,
-                                     uint32_t blk, bool det) {
+__device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t p,
+                                           uint32_t slot, uint32_t blk, bool det) {
   uint32_t* sp = s->slots + s->slot_off[t] + slot;
   uint32_t seen = 0;
   if (wave::is_first()) seen = atomic_cas(sp, blk, kNullBlock);
   if (wave::readfirst(seen) != blk) return;
-  const uint32_t nb = request_new_block(s, t, det);
+  const uint32_t nb = request_new_block(s, t, p, det);
   if (nb == kNullBlock) return;
-  uint32_t prev = 0;
   if (wave::is_first()) {
-    utils::fence_release();
-    prev = atomic_cas(sp, kNullBlock, nb);
-    if (prev != kNullBlock) {
-      free_block(s, nb);
+    if (atomic_cas(sp, kNullBlock, nb) != kNullBlock) {
+      free_block(s, nb, p);
       add_stat(s, kStatErrors, 1);
     }
   }
@@ -357,7 +390,8 @@ __device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t sl
 // Slice phase of malloc: gallatin.cuh:865-1078 malloc_slice_allocation for
 // every size class present in the wave at once.
 __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t t,
-                                     uint32_t cnt, uint64_t& out_off) {
+                                              uint32_t cnt, uint32_t p,
+                                              uint64_t& out_off) {
   const bool det = deterministic(s);
   const uint32_t lane = wave::lane();
   bool pending = want;
@@ -369,10 +403,10 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
     const uint64_t grp = wave::match(pend, pending, t);
     const int leader = pending ? wave::first(grp) : (int)lane;
     const bool is_leader = pending && leader == (int)lane;
-    uint32_t slot = 0, blk = kNullBlock;
-    if (is_leader) blk = pinned_block(s, t, slot_start(s, t), slot);
-    slot = wave::fetch(slot, leader);
-    blk = wave::fetch(blk, leader);
+    uint64_t sb = kNullBlock;
+    if (is_leader) sb = pinned_block(s, t, p);
+    sb = wave::fetch(sb, leader);
+    const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
     const bool have = pending && blk != kNullBlock;
     // exclusive prefix in lane order and group total (block.cuh:245-264)
     uint32_t prefix, total;
@@ -404,9 +438,8 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
       for (uint64_t m = __ballot(completes); m; m &= m - 1) {
         const uint32_t b = wave::readlane(blk, wave::first(m));
         if (wave::is_first()) {
-          utils::fence_acquire();
           atomic_exch(&block_at(s, b)->free_counter, 0u);
-          free_block(s, b);
+          free_block(s, b, p);
         }
       }
     }
@@ -414,19 +447,20 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
     const uint64_t rep = __ballot(should_replace);
     for (uint64_t m = __ballot(is_leader && (grp & rep) != 0); m; m &= m - 1) {
       const int l = wave::first(m);
-      replace_block(s, wave::readlane(t, l), wave::readlane(slot, l),
+      replace_block(s, wave::readlane(t, l), p, wave::readlane(slot, l),
                     wave::readlane(blk, l), det);
     }
     // check_valid (block.cuh:343-357): the block still belongs to class t
+    // (and to this partition)
     const bool valid = have && end_valid;
-    const bool ok = valid && (old >> kTreeShift) == t;
+    const bool ok = valid && (old >> kTreeShift) == block_tag(t, p);
     if (ok) {
       out_off = (uint64_t)al + ((uint64_t)blk << 12);
       pending = false;
     }
     for (uint64_t m = __ballot(valid && !ok); m; m &= m - 1) {
       const uint32_t b = wave::readlane(blk, wave::first(m));
-      if (wave::is_first()) block_release(s, b, 1);
+      if (wave::is_first()) block_release(s, b, 1, p);
     }
     if (pending && ++attempts >= s->lim_slice) pending = false;
   }
@@ -434,20 +468,21 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
 
 // gallatin.cuh:828-861 malloc_block_allocation + block.cuh:288-340
 // malloc_fill_block, retried as gallatin.cuh:1226-1231.  Byte offset.
-__device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, bool det) {
+__device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, uint32_t p,
+                                                   bool det) {
   for (uint32_t attempt = 0; attempt < s->lim_block; ++attempt) {
-    const uint32_t blk = request_new_block(s, bt, det);
+    const uint32_t blk = request_new_block(s, bt, p, det);
     if (blk == kNullBlock) continue;
     uint32_t res = 0;
     if (wave::is_first()) {
       const uint32_t merged = atomic_add(&block_at(s, blk)->malloc_counter, 4096u);
       const uint32_t oldc = merged & kCountMask;
-      if ((merged >> kTreeShift) != bt) {  // check_valid -> trap
+      if ((merged >> kTreeShift) != block_tag(bt, p)) {  // check_valid -> trap
         add_stat(s, kStatErrors, 1);
       } else {
         atomic_add(&block_at(s, blk)->free_counter, 4095u - oldc);
         if (oldc != 0)
-          block_release(s, blk, 1);
+          block_release(s, blk, 1, p);
         else
           res = 1;
       }
@@ -466,7 +501,7 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
   for (uint32_t attempt = 0; attempt < s->lim_segment; ++attempt) {
     bool got = false;
     for (uint32_t spin = 0; spin < kSpinCap && !got; ++spin) {
-      got = try_lock(s, T);
+      got = try_lock(s, 0);
       if (!got) utils::backoff();
     }
     if (!got) return kFail;
@@ -477,8 +512,8 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
       uint64_t cur = atomic_load(&s->seg_state[idx]);
       for (int g = 0; g < 4096; ++g) {
         if ((cur & 0xFFFF) != kFreeTree) break;
-        const uint64_t seen =
-            atomic_cas(&s->seg_state[idx], cur, (uint64_t)((cur & ~0xFFFFULL) | (T + 1 + n)));
+        const uint64_t seen = atomic_cas(&s->seg_state[idx], cur,
+                                         (uint64_t)((cur & ~kSegIdMask) | (T + 1 + n)));
         if (seen == cur) {
           ok = 1;
           break;
@@ -491,7 +526,7 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
       }
     }
     ok = wave::readfirst(ok);
-    unlock(s, T);
+    unlock(s, 0);
     if (ok) return idx << s->seg_shift;
   }
   return kFail;
@@ -502,6 +537,7 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
   const bool det = deterministic(s);
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
+  const uint32_t p = wave_part(s);
   // size dispatch (gallatin.cuh:1141-1147, 1196-1258)
   const int cl = utils::get_first_bit_bigger(size < 1 ? 1 : size);
   uint32_t t = size < s->smallest ? 0u : (uint32_t)(cl - (int)s->smallest_bits);
@@ -524,12 +560,12 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
   uint64_t bytes = kFail;
   if (__ballot(path == kPathSlice)) {
     uint64_t off = kFail;
-    malloc_slices(s, path == kPathSlice, t, cnt, off);
+    malloc_slices(s, path == kPathSlice, t, cnt, p, off);
     if (off != kFail) bytes = offset_to_bytes(s, off, t);
   }
   for (uint64_t m = __ballot(path == kPathBlock); m; m &= m - 1) {
     const int l = wave::first(m);
-    const uint64_t r = malloc_block_path(s, wave::readlane(t, l), det);
+    const uint64_t r = malloc_block_path(s, wave::readlane(t, l), p, det);
     if ((int)lane == l) bytes = r;
   }
   for (uint64_t m = __ballot(path == kPathSegment); m; m &= m - 1) {
@@ -537,7 +573,11 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
     const uint64_t r = malloc_segment_path(s, wave::readlane(nseg, l));
     if ((int)lane == l) bytes = r;
   }
-  if (!(s->flags & kFlagNoPayloadFence) && __ballot(bytes != kFail))
+  // With one partition a slice may come from any XCD's past: acquire at
+  // agent scope.  With XCD partitions the previous users of a slice wrote
+  // through this XCD's L2 or were written back before their free.
+  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1 &&
+      __ballot(bytes != kFail))
     utils::fence_acquire();
   return bytes == kFail ? nullptr : (void*)(s->memory + bytes);
 }
@@ -547,21 +587,29 @@ __device__ __forceinline__ void free_impl(gal_state* s, void* ptr) {
   const bool det = deterministic(s);
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
+  const uint32_t my_p = wave_part(s);
   const uint64_t base = (uint64_t)s->memory;
   const uint64_t p = (uint64_t)ptr;
   const bool in = ptr != nullptr && (p - base) < (s->num_segments << s->seg_shift);
-  uint64_t seg = 0;
+  uint64_t seg = 0, st = 0;
   uint32_t t = kFreeTree;
   if (in) {
     seg = (p - base) >> s->seg_shift;
-    t = (uint32_t)(atomic_load(&s->seg_state[seg]) & 0xFFFF);
+    st = atomic_load(&s->seg_state[seg]);
+    t = (uint32_t)(st & 0xFFFF);
   }
   const bool multi = in && t > T && t != kFreeTree;
   const bool slice = in && t < T;
   const uint64_t bad = __ballot(ptr != nullptr && !multi && !slice);
   if (bad && wave::is_first()) add_stat(s, kStatErrors, wave::count(bad));
   if (!__ballot(multi || slice)) return;
-  if (!(s->flags & kFlagNoPayloadFence)) utils::fence_release();
+  if (!(s->flags & kFlagNoPayloadFence)) {
+    // a free away from the segment's home XCD writes this XCD's L2 back
+    if (s->num_parts == 1 || __ballot(multi || (slice && seg_home(st) != my_p)))
+      utils::fence_release();
+    else
+      release_to_xcd();
+  }
   // multi-segment frees one at a time in lane order
   for (uint64_t m = __ballot(multi); m; m &= m - 1) {
     const int l = wave::first(m);
@@ -599,9 +647,8 @@ __device__ __forceinline__ void free_impl(gal_state* s, void* ptr) {
   for (uint64_t m = __ballot(completes); m; m &= m - 1) {
     const uint32_t b = wave::readlane(blk, wave::first(m));
     if (wave::is_first()) {
-      utils::fence_acquire();
       atomic_exch(&block_at(s, b)->free_counter, 0u);
-      free_block(s, b);
+      free_block(s, b, my_p);
     }
   }
 }

@@ -18,6 +18,7 @@
 #ifndef GALLATIN_LAYOUT_HPP
 #define GALLATIN_LAYOUT_HPP
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace gallatin {
@@ -31,6 +32,7 @@ constexpr uint32_t kNullBlock = 0xFFFFFFFFu;
 constexpr uint64_t kFail = ~0ULL;
 constexpr int kMaxLayers = 4;                // universes up to 2^24 segments
 constexpr int kMaxTrees = 24;
+constexpr int kMaxParts = 8;                 // XCD partitions
 
 // retry limits (gallatin.cuh:108-112, shared_block_storage.cuh:31)
 constexpr int kRequestBlockAttempts = 1000;
@@ -45,9 +47,21 @@ constexpr uint32_t kSpinCap = 1u << 16;
 constexpr uint32_t kFlagDeterministic = 1u;  // smallest-id segment, slot 0
 constexpr uint32_t kFlagNoPayloadFence = 2u; // skip release/acquire on payload
 
-// seg_state packing
+// seg_state packing: bits 0-15 tree id (kFreeTree = unformatted), bits
+// 16-23 home partition, bits 32-63 active count (int32)
 constexpr uint64_t kActiveOne = 1ULL << 32;
+constexpr uint64_t kSegIdMask = 0xFFFFFFULL;
 constexpr uint64_t kSegFreeState = 0xFFFFFFFF0000FFFFULL;  // active -1, tree free
+
+__host__ __device__ inline uint32_t seg_home(uint64_t st) { return (uint32_t)(st >> 16) & 0xFFu; }
+__host__ __device__ inline uint64_t seg_id(uint32_t tree, uint32_t home) {
+  return (uint64_t)tree | ((uint64_t)home << 16);
+}
+// Tag stored above the count of a block's malloc_counter: class and home
+// partition (just the class with one partition, as in block.cuh:215-224).
+__host__ __device__ inline uint32_t block_tag(uint32_t tree, uint32_t home) {
+  return tree | (home << 5);
+}
 
 struct gal_block {
   uint32_t malloc_counter;  // (tree << 20) | claimed slices
@@ -111,6 +125,8 @@ struct gal_state {
   uint32_t flags;
   uint32_t total_slots;
   uint32_t block_bits;           // block array holds 2^block_bits entries
+  uint32_t num_parts;            // XCD partitions (1 in deterministic mode)
+  uint32_t pad1;
   // retry limits (defaults kRequestBlockAttempts, kSliceAttempts, ...)
   uint32_t lim_request;
   uint32_t lim_slice;
@@ -123,12 +139,14 @@ struct gal_state {
   uint32_t* ring;                // [N * B] recycled block ids
   gal_block* blocks;             // [2^block_bits], at block_position()
   uint32_t* slots;               // pinned blocks of all size classes
-  uint32_t* locks;               // [1] tree lock bits (gallatin.cuh:314)
+  uint32_t* locks;               // [1 + T * parts]: multi-segment lock, then one
+                                 // per class tree (gallatin.cuh:314, 1513-1519)
   unsigned long long* stats;     // [kStatCount]
   uint32_t slot_off[kMaxTrees + 1];
-  uint32_t slot_count[kMaxTrees];
+  uint32_t slot_count[kMaxTrees];  // multiple of num_parts; partition p owns
+                                   // slots [p, p+1) * count / num_parts
   gal_tree seg_tree;
-  gal_tree sub_trees[kMaxTrees];
+  gal_tree sub_trees[kMaxTrees * kMaxParts];  // class t, partition p at t*parts+p
 };
 
 }  // namespace allocators

@@ -53,6 +53,11 @@ typedef struct gal_config {
   uint32_t slice_attempts;
   uint32_t block_attempts;
   uint32_t segment_attempts;
+  /* XCD partitions of the pinned slots and class trees: 0 = one per XCD of
+   * the device (8 on MI355X); forced to 1 in deterministic mode.  Slot
+   * counts are rounded up to a multiple of it. */
+  uint32_t partitions;
+  uint32_t pad;
 } gal_config;
 
 typedef struct gal_allocator gal_allocator;
@@ -61,6 +66,7 @@ typedef struct gal_info {
   uint64_t num_segments, bytes_per_segment, smallest, biggest;
   uint64_t blocks_per_segment;
   uint32_t num_trees, total_slots;
+  uint32_t partitions, pad;
   uint64_t segments_free;          /* bits set in the segment tree */
   uint64_t segments_formatted;     /* segments owned by a size class */
   uint64_t segments_multi;         /* heads of multi-segment allocations */
@@ -112,11 +118,11 @@ int gal_reboot(gal_allocator *a, void *stream);
 int gal_snapshot(gal_allocator *a, uint64_t *seg_state, uint32_t *slots,
                  uint32_t *blocks);
 
-/* Tree words: tree -1 = segment tree, 0..T-1 = size-class trees.  Writes
- * the words of all layers, top layer first, and their counts.  Returns the
- * number of words needed when out is NULL. */
-int64_t gal_tree_words(gal_allocator *a, int tree, uint64_t *out, uint64_t cap,
-                       uint32_t *num_layers, uint64_t *layer_words);
+/* Tree words: tree -1 = segment tree, 0..T-1 = size-class trees (of XCD
+ * partition `partition`).  Writes the words of all layers, top layer first,
+ * and their counts.  Returns the number of words needed when out is NULL. */
+int64_t gal_tree_words(gal_allocator *a, int tree, int partition, uint64_t *out,
+                       uint64_t cap, uint32_t *num_layers, uint64_t *layer_words);
 
 /* ---- global instance (global_allocator.cuh:41-97) ---- */
 /* init_global_allocator / free_global_allocator / print_global_stats for the

@@ -49,8 +49,9 @@ def check_fresh(a):
     for x, y in zip(a.tree_layers(-1), H.fresh_layers(a.num_segments, True)):
         assert np.array_equal(x, y)
     for t in range(a.num_trees):
-        for x in a.tree_layers(t):
-            assert not np.any(x)
+        for p in range(a.partitions):
+            for x in a.tree_layers(t, p):
+                assert not np.any(x)
     assert a.info()["errors"] == 0
 
 
