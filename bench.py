@@ -7,14 +7,16 @@ Workload (one step = one pass of the hot path over one batch):
   live allocation with probability 1/2; every allocation is filled with a
   uint64 (thread, round) pattern and verified before it is freed.
 
-Prints one JSON line (rank 0).  `value` is successful malloc+free pairs per
-second over the timed steps, counted on the device; the roofline is that of
-the churn kernel (payload bytes written and read against HBM peak, and the
-atomic issue rate against the measured atomic ceiling); `cpu_baseline` is the
-oracle's sequential replay of the same request streams on one host core, with
-multithreaded glibc malloc/free of the same streams beside it.
+Both paths run the same request streams: the batched path (workgroup-batched
+malloc/free in one persistent launch, the headline `value`) and the plain
+per-thread path (`plain` beside it).  Pairs are counted on the device.  The
+roofline is the headline kernel's: payload bytes written and read against HBM
+peak, and its atomic issue rate against the measured atomic ceiling.
+`cpu_baseline` is the oracle's sequential replay of the same request streams
+on one host core; multithreaded glibc malloc/free of the same streams is
+reported beside it.
 
-  python bench.py [--steps K] [--warmup W] [--threads N] [--no-cpu-baseline]
+  python bench.py [--steps K] [--warmup W] [--path batched|plain|both]
 """
 import argparse
 import ctypes
@@ -36,6 +38,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--path", choices=("batched", "plain", "both"), default="both")
     p.add_argument("--threads", type=int, default=1 << 24)
     p.add_argument("--rounds", type=int, default=10)
     p.add_argument("--pool-gib", type=int, default=8)
@@ -45,6 +48,8 @@ def parse():
     p.add_argument("--partitions", type=int, default=0)
     p.add_argument("--flags", type=int, default=0)
     p.add_argument("--block-threads", type=int, default=256)
+    p.add_argument("--workgroups", type=int, default=0)
+    p.add_argument("--batch-threads", type=int, default=256)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-ceilings", action="store_true")
     p.add_argument("--cpu-sample-threads", type=int, default=3 << 19)
@@ -61,13 +66,15 @@ def host_cores():
 
 def atomic_ceiling(G, torch):
     """Agent-scope atomicAdd(u32) with returned value: distinct consecutive
-    addresses, one address per 128-byte line, one address per wave."""
+    addresses, one address per 128-byte line, one address per wave; and a
+    16-byte-vector copy for the HBM ceiling."""
     words = torch.zeros(1 << 26, dtype=torch.int32, device="cuda")
     out = {}
     for name, span, lanes, stride, nops in (
             ("distinct", 1 << 26, 1, 1, 1 << 28),
             ("per_line", 1 << 26, 1, 32, 1 << 27),
-            ("per_wave", 1 << 26, 64, 32, 1 << 26)):
+            ("per_wave", 1 << 26, 64, 32, 1 << 26),
+            ("one_address", 1, 1, 1, 1 << 22)):
         G.bench_atomic(words, span, nops, lanes, stride)  # warm
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -125,6 +132,52 @@ def cpu_baselines(args):
     return res
 
 
+def run_path(args, G, torch, path):
+    """Warm up, then time exactly args.steps launches of one path."""
+    alloc = G.Gallatin(16 << 20, 16, 4096).generate_on_device(
+        args.pool_gib << 30, args.seed, print_info=False, flags=args.flags,
+        slots0=args.slots0, slots_min=args.slots_min, partitions=args.partitions)
+    ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
+
+    def step(seed):
+        if path == "batched":
+            alloc.run_churn_batched(seed, args.threads, args.rounds, counters=ctr,
+                                    workgroups=args.workgroups, threads=args.batch_threads)
+        else:
+            alloc.run_churn(seed, args.threads, args.rounds, counters=ctr,
+                            block_threads=args.block_threads)
+
+    for w in range(args.warmup):
+        step(args.seed + w)
+    torch.cuda.synchronize()
+    ctr.zero_()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record()
+    for k in range(args.steps):
+        step(args.seed + 100 + k)
+        evs[k + 1].record()
+    torch.cuda.synchronize()
+    total_ms = evs[0].elapsed_time(evs[-1])
+    step_ms = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
+    c = ctr.cpu().tolist()
+    info = alloc.info()
+    alloc.free_on_device()
+    pairs = c[G.CTR_PAIRS]
+    return {
+        "value": pairs / (total_ms / 1e3),
+        "ms_per_step": total_ms / args.steps,
+        "p50_step_ms": step_ms[len(step_ms) // 2],
+        "p99_step_ms": step_ms[min(len(step_ms) - 1, int(round(0.99 * (len(step_ms) - 1))))],
+        "pairs_per_step": pairs / args.steps,
+        "bytes_per_step": c[G.CTR_BYTES] / args.steps,
+        "misses": c[G.CTR_MISSES],
+        "violations": c[G.CTR_BAD] + info["errors"],
+        "xcd_partitions": info["partitions"],
+        "kernel": "k_churn_batched" if path == "batched" else "k_churn",
+    }
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -137,40 +190,18 @@ def main():
     import gallatin_amd as G
 
     torch.cuda.set_device(0)
-    alloc = G.Gallatin(16 << 20, 16, 4096).generate_on_device(
-        args.pool_gib << 30, args.seed, print_info=False, flags=args.flags,
-        slots0=args.slots0, slots_min=args.slots_min, partitions=args.partitions)
-    ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
-    for w in range(args.warmup):
-        alloc.run_churn(args.seed + w, args.threads, args.rounds, counters=ctr,
-                        block_threads=args.block_threads)
-    torch.cuda.synchronize()
-    ctr.zero_()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    evs[0].record()
-    for k in range(args.steps):
-        alloc.run_churn(args.seed + 100 + k, args.threads, args.rounds, counters=ctr,
-                        block_threads=args.block_threads)
-        evs[k + 1].record()
-    torch.cuda.synchronize()
-    total_ms = evs[0].elapsed_time(evs[-1])
-    step_ms = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps))
-    c = ctr.cpu().tolist()
-    info = alloc.info()
-    partitions = alloc.partitions
-    pairs, misses, bad, nbytes = c[G.CTR_PAIRS], c[G.CTR_MISSES], c[G.CTR_BAD], c[G.CTR_BYTES]
-    value = pairs / (total_ms / 1e3)
-    avg_ms = total_ms / args.steps
-    achieved_gbs = nbytes / args.steps / (avg_ms / 1e3) / 1e9
+    paths = ["batched", "plain"] if args.path == "both" else [args.path]
+    res = {p: run_path(args, G, torch, p) for p in paths}
+    head = res[paths[0]]
+    achieved_gbs = head["bytes_per_step"] / (head["ms_per_step"] / 1e3) / 1e9
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": avg_ms,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -179,21 +210,21 @@ def main():
         "config": {
             "workload": "mixed-size churn: 8 GiB pool, 2^24 threads x 10 rounds, "
                         "log-uniform 1..4095 B, 50% replace per round, fill+verify",
-            "path": "plain per-thread malloc/free",
+            "path": paths[0],
             "allocator": "Gallatin<16 MiB, 16 B, 4096 B>",
             "pool_bytes": args.pool_gib << 30,
             "threads": args.threads,
             "rounds": args.rounds,
             "slots0": args.slots0,
             "slots_min": args.slots_min,
-            "xcd_partitions": partitions,
-            "block_threads": args.block_threads,
+            "xcd_partitions": head["xcd_partitions"],
+            "flags": args.flags,
         },
-        "p50_step_ms": step_ms[len(step_ms) // 2],
-        "p99_step_ms": step_ms[min(len(step_ms) - 1, int(round(0.99 * (len(step_ms) - 1))))],
-        "pairs_per_step": pairs / args.steps,
-        "misses": misses,
-        "violations": bad + info["errors"],
+        "p50_step_ms": head["p50_step_ms"],
+        "p99_step_ms": head["p99_step_ms"],
+        "pairs_per_step": head["pairs_per_step"],
+        "misses": head["misses"],
+        "violations": head["violations"],
         "roofline": {
             "bound": "hbm",
             "achieved": achieved_gbs,
@@ -201,18 +232,21 @@ def main():
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_churn",
-            "algorithmic_bytes_per_launch": nbytes / args.steps,
+            "kernel": head["kernel"],
+            "algorithmic_bytes_per_launch": head["bytes_per_step"],
         },
     }
-    alloc.free_on_device()
+    if "plain" in res and paths[0] != "plain":
+        pl = res["plain"]
+        out["plain"] = {k: pl[k] for k in ("value", "ms_per_step", "p50_step_ms",
+                                            "p99_step_ms", "misses", "violations", "kernel")}
+        out["batched_over_plain"] = head["value"] / pl["value"]
     if not args.no_ceilings:
         ceil = atomic_ceiling(G, torch)
-        atoms = 2.0 * value  # one claim + one release per pair
+        atoms = 2.0 * head["value"]  # one claim + one release per pair
         out["atomic_roofline"] = {
             "achieved": atoms, "peak": ceil["distinct"], "unit": "atomics/s",
-            "frac": atoms / ceil["distinct"], "per_pair": 2,
-            "ceilings": ceil,
+            "frac": atoms / ceil["distinct"], "per_pair": 2, "ceilings": ceil,
         }
     if not args.no_cpu_baseline:
         out.update(cpu_baselines(args))

@@ -41,7 +41,8 @@ CAPI_SYMBOLS = [
     "gal_free_global", "gal_print_global_stats", "gal_global_handle",
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
-    "gal_run_churn", "gal_bench_atomic", "gal_bench_copy", "gal_last_error",
+    "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
+    "gal_last_error",
 ]
 
 
@@ -105,6 +106,7 @@ def lib():
         L.gal_run_wave_stream.argtypes = [_p, _p, _u64, _p, _p]
         L.gal_run_plain.argtypes = [_p, _u64, _u64, _p, _u32, _p]
         L.gal_run_churn.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _p]
+        L.gal_run_churn_batched.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _p]
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
         L.gal_last_error.restype = ctypes.c_char_p
@@ -269,6 +271,13 @@ class Gallatin:
         ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
         _check(lib().gal_run_churn(self.handle, seed, nthreads, rounds,
                                    ctr.data_ptr(), block_threads, _stream()))
+        return ctr
+
+    def run_churn_batched(self, seed, nthreads, rounds, counters=None, workgroups=0,
+                          threads=256):
+        ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+        _check(lib().gal_run_churn_batched(self.handle, seed, nthreads, rounds,
+                                           ctr.data_ptr(), workgroups, threads, _stream()))
         return ctr
 
 

@@ -220,7 +220,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
 
   gal_allocator* a = new gal_allocator();
   std::memset(&a->host, 0, sizeof(a->host));
-  hipGetDevice(&a->device);
+  (void)hipGetDevice(&a->device);
   gal_state& h = a->host;
   h.bytes_per_segment = S;
   h.smallest = m;
@@ -300,7 +300,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   a->pool_bytes = N * S;
   e = hipMalloc(&a->pool, a->pool_bytes);
   if (e != hipSuccess) {
-    hipFree(a->meta);
+    (void)hipFree(a->meta);
     delete a;
     return hip_fail(e, "hipMalloc(pool)");
   }
@@ -339,12 +339,16 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   }
   if (rc == GAL_OK) {
     unsigned long long boot_failed = 0;
-    hipMemcpy(&boot_failed, h.stats + kStatBootFailed, sizeof(boot_failed), hipMemcpyDeviceToHost);
-    if (boot_failed) rc = fail(GAL_ERR_BOOT, "pool too small for the pinned boot blocks");
+    const hipError_t e = hipMemcpy(&boot_failed, h.stats + kStatBootFailed,
+                                   sizeof(boot_failed), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+      rc = hip_fail(e, "read boot status");
+    else if (boot_failed)
+      rc = fail(GAL_ERR_BOOT, "pool too small for the pinned boot blocks");
   }
   if (rc != GAL_OK) {
-    hipFree(a->pool);
-    hipFree(a->meta);
+    (void)hipFree(a->pool);
+    (void)hipFree(a->meta);
     delete a;
     return rc;
   }
@@ -397,7 +401,7 @@ int gal_reboot(gal_allocator* a, void* stream) {
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "reboot");
   unsigned long long boot_failed = 0;
-  hipMemcpy(&boot_failed, a->host.stats + kStatBootFailed, sizeof(boot_failed), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(&boot_failed, a->host.stats + kStatBootFailed, sizeof(boot_failed), hipMemcpyDeviceToHost);
   if (boot_failed) return fail(GAL_ERR_BOOT, "reboot ran out of segments");
   return GAL_OK;
 }

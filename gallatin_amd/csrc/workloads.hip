@@ -7,6 +7,7 @@
 // tests/src/gallatin_churn.cu:37-95 (pointer_churn_kernel).
 #include <hip/hip_runtime.h>
 
+#include "gallatin/allocators/batched.hpp"
 #include "gallatin/allocators/gallatin.hpp"
 #include "gallatin/streams.h"
 #include "gallatin_capi.h"
@@ -221,6 +222,54 @@ __global__ void k_churn(gal_state* s, uint64_t seed, uint64_t n, uint32_t rounds
   add_counters(ctr, c);
 }
 
+// The same churn through the workgroup-batched path: a persistent grid walks
+// chunks of blockDim stream threads; for every round the workgroup frees its
+// replaced allocations as one batch and allocates the new ones as one batch.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_churn_batched(gal_state* s, uint64_t seed,
+                                                            uint64_t n, uint32_t rounds,
+                                                            uint64_t* ctr) {
+  __shared__ batch::lds_state lds;
+  batch::batch_init(lds);
+  lane_counts c;
+  const uint64_t nchunks = (n + blockDim.x - 1) / blockDim.x;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t tid = chunk * blockDim.x + threadIdx.x;
+    const bool active = tid < n;
+    uint64_t* live = nullptr;
+    uint32_t live_words = 0;
+    uint32_t live_round = 0;
+    for (uint32_t r = 0; r <= rounds; ++r) {
+      const bool replace =
+          active && (r == 0 || r == rounds || gal_churn_coin(seed, tid, r) != 0);
+      const bool do_free = replace && live != nullptr;
+      c.bad += wave_verify(do_free, live, live_words, gal_churn_pattern(tid, live_round));
+      batch::batch_free(s, do_free ? live : nullptr, lds);
+      if (do_free) {
+        c.pairs += 1;
+        c.bytes += (uint64_t)live_words * 8;
+        live = nullptr;
+      }
+      if (r == rounds) break;
+      const uint32_t size = (uint32_t)gal_churn_size(seed, tid, r);
+      uint64_t* np = static_cast<uint64_t*>(batch::batch_malloc(s, replace, size, lds));
+      if (replace) {
+        c.mallocs += 1;
+        if (!np) c.misses += 1;
+      }
+      const bool filled = replace && np != nullptr;
+      if (filled) {
+        live = np;
+        live_words = (uint32_t)gal_fill_words(size);
+        live_round = r;
+        c.bytes += (uint64_t)live_words * 8;
+      }
+      wave_fill(filled, np, gal_fill_words(size), gal_churn_pattern(tid, r));
+    }
+  }
+  add_counters(ctr, c);
+}
+
 // Atomic-rate probe: op i adds 1 to word ((i / lanes_per_address) * stride)
 // % span and keeps the returned value live, as the allocator's claims do.
 __global__ void k_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,
@@ -247,6 +296,25 @@ static unsigned grid_for(uint64_t n, uint32_t threads) {
 static int launch_status(const char* what) {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? GAL_OK : hip_fail(e, what);
+}
+
+template <int kThreads>
+static int launch_churn_batched(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,
+                                uint64_t* ctr, uint32_t workgroups, hipStream_t stream) {
+  if (workgroups == 0) {
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_churn_batched<kThreads>,
+                                                     kThreads, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    workgroups = (uint32_t)(cus * per_cu);
+  }
+  const uint64_t chunks = (n + kThreads - 1) / kThreads;
+  if (workgroups > chunks) workgroups = (uint32_t)chunks;
+  k_churn_batched<kThreads><<<workgroups, kThreads, 0, stream>>>(a->dev, seed, n, rounds, ctr);
+  return launch_status("churn_batched");
 }
 
 extern "C" {
@@ -293,6 +361,21 @@ int gal_run_churn(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,
   if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
   k_churn<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, seed, n, rounds, ctr);
   return launch_status("churn");
+}
+
+int gal_run_churn_batched(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,
+                          uint64_t* ctr, uint32_t workgroups, uint32_t threads,
+                          void* stream) {
+  if (!a || !ctr) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  hipStream_t st = (hipStream_t)stream;
+  switch (threads) {
+    case 0:
+    case 256: return launch_churn_batched<256>(a, seed, n, rounds, ctr, workgroups, st);
+    case 512: return launch_churn_batched<512>(a, seed, n, rounds, ctr, workgroups, st);
+    case 1024: return launch_churn_batched<1024>(a, seed, n, rounds, ctr, workgroups, st);
+    default: return fail(GAL_ERR_INVALID, "batched churn runs 256, 512 or 1024 threads per workgroup");
+  }
 }
 
 int gal_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,

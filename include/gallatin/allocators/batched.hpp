@@ -1,0 +1,207 @@
+// Workgroup-batched malloc / free: the requests of a whole workgroup are
+// bucketed in LDS and served together.
+//
+//   batch_malloc: requests are counted per size class in LDS; one wave then
+//     claims every class's whole count at once, lane c for class c, with one
+//     counter atomic per class (more only when a claim runs past slice 4095
+//     and the block is replaced), and each request takes its slice by its
+//     rank in the class.  This is the reference's coalesced group claim
+//     (gallatin.cuh:865-1078, block.cuh:245-264) with the group widened from
+//     a warp to a workgroup.
+//   batch_free: requests are grouped per block in an LDS hash table and each
+//     block gets one free-counter atomic (gallatin.cuh:1774-1827 with a
+//     workgroup-wide team).
+//
+// Requests the batch cannot serve (whole blocks, multi-segment runs, runs of
+// 4 KiB slices, deterministic mode, or a class that needs more than kRanges
+// claims) go through the per-wave path of gallatin.hpp, so results and
+// failure behaviour are those of malloc / free.
+//
+// Every thread of the workgroup must call these functions (workgroup-uniform
+// control flow; blockDim.x a multiple of 64 and at most 1024), and the LDS
+// state must be set up once with batch_init before the first call.
+#ifndef GALLATIN_BATCHED_HPP
+#define GALLATIN_BATCHED_HPP
+
+#include "gallatin.hpp"
+
+namespace gallatin {
+namespace allocators {
+namespace batch {
+
+constexpr int kRanges = 4;        // claims per class per batch
+constexpr int kFreeSlots = 2048;  // LDS hash slots for block grouping
+
+struct lds_state {
+  // malloc
+  uint32_t count[kMaxTrees];
+  uint32_t nranges[kMaxTrees];
+  uint32_t blk[kMaxTrees][kRanges];
+  uint32_t start[kMaxTrees][kRanges];
+  uint32_t len[kMaxTrees][kRanges];
+  uint32_t first[kMaxTrees][kRanges];
+  // free
+  uint32_t key[kFreeSlots];  // block id + 1, 0 = empty
+  uint32_t cnt[kFreeSlots];
+};
+
+// Clear the LDS state (whole workgroup, once per kernel).
+__device__ inline void batch_init(lds_state& L) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kFreeSlots; i += blockDim.x) L.key[i] = 0, L.cnt[i] = 0;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxTrees; i += blockDim.x) L.count[i] = 0;
+  __syncthreads();
+}
+
+// Claims of all classes with pending requests, lane c for class c (one
+// wave, uniform).  A class whose claim ran out of its block retries in lane
+// order until served, out of ranges or out of attempts.
+__device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
+  const uint32_t T = s->num_trees;
+  const uint32_t c = wave::lane();
+  const uint32_t need = c < T ? L.count[c] : 0u;
+  if (c < T) L.count[c] = 0;  // ready for the next batch
+  const bool mine = need != 0;
+  uint32_t served = 0, nr = 0, attempts = 0;
+  bool open = mine;
+  while (__ballot(open)) {
+    uint64_t sb = kNullBlock;
+    if (open) sb = detail::pinned_block(s, c, p);
+    const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
+    const bool have = open && blk != kNullBlock;
+    uint32_t old = 0;
+    if (have) old = utils::atomic_add(&detail::block_at(s, blk)->malloc_counter, need - served);
+    const uint32_t cnt = old & kCountMask;
+    const bool tag_ok = have && (old >> kTreeShift) == block_tag(c, p);
+    uint32_t take = 0;
+    if (have && cnt < 4096u) take = min(need - served, 4096u - cnt);
+    if (have && !tag_ok && take) detail::block_release(s, blk, take, p);  // stale block
+    const bool got = tag_ok && take > 0;
+    if (got) {
+      L.blk[c][nr] = blk;
+      L.start[c][nr] = cnt;
+      L.len[c][nr] = take;
+      L.first[c][nr] = served;
+      ++nr;
+      served += take;
+    }
+    // the claim that took slice 4095 replaces the block, one class at a time
+    const uint64_t rep = __ballot(got && cnt + take == 4096u);
+    for (uint64_t m = rep; m; m &= m - 1) {
+      const int l = wave::first(m);
+      detail::replace_block(s, wave::readlane(c, l), p, wave::readlane(slot, l),
+                            wave::readlane(blk, l), false);
+    }
+    if (open && !(served == need || nr == (uint32_t)kRanges || ++attempts >= s->lim_slice))
+      continue;
+    open = false;
+  }
+  if (c < T) L.nranges[c] = nr;
+}
+
+// Each thread with `want` gets a pointer of `size` bytes (or nullptr).
+__device__ inline void* batch_malloc(gal_state* s, bool want, uint64_t size, lds_state& L) {
+  const uint32_t T = s->num_trees;
+  const uint32_t p = detail::wave_part(s);
+  uint32_t t = 0;
+  bool batched = false;
+  if (want && !detail::deterministic(s)) {
+    t = size < s->smallest ? 0u
+                           : (uint32_t)(utils::get_first_bit_bigger(size) - (int)s->smallest_bits);
+    batched = t < T;
+  }
+  uint32_t rank = 0;
+  if (batched) rank = atomicAdd(&L.count[t], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64) claim_classes(s, L, p);
+  __syncthreads();
+  uint64_t bytes = kFail;
+  if (batched) {
+    for (uint32_t r = 0; r < L.nranges[t]; ++r) {
+      if (rank >= L.first[t][r] && rank < L.first[t][r] + L.len[t][r]) {
+        const uint64_t off = (uint64_t)L.start[t][r] + (rank - L.first[t][r]) +
+                             ((uint64_t)L.blk[t][r] << 12);
+        bytes = detail::offset_to_bytes(s, off, t);
+        break;
+      }
+    }
+  }
+  // (the ranges are rewritten only after the next batch's first barrier,
+  // which every thread reaches after reading them here)
+  // everything else through the wave path
+  void* out = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
+  const bool rest = want && bytes == kFail;
+  if (__ballot(rest)) {
+    void* q = nullptr;
+    if (rest) q = detail::malloc_impl(s, size);
+    if (rest) out = q;
+  }
+  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
+  return out;
+}
+
+// Each thread frees `ptr` (nullptr: nothing).
+__device__ inline void batch_free(gal_state* s, void* ptr, lds_state& L) {
+  const uint32_t T = s->num_trees;
+  const uint32_t my_p = detail::wave_part(s);
+  const uint64_t base = (uint64_t)s->memory;
+  const uint64_t pa = (uint64_t)ptr;
+  bool slice = false, foreign = false;
+  uint32_t blk = 0;
+  if (ptr != nullptr && !detail::deterministic(s) &&
+      (pa - base) < (s->num_segments << s->seg_shift)) {
+    const uint64_t seg = (pa - base) >> s->seg_shift;
+    const uint64_t st = utils::atomic_load(&s->seg_state[seg]);
+    const uint32_t t = (uint32_t)(st & 0xFFFF);
+    if (t < T) {
+      slice = true;
+      blk = (uint32_t)(detail::bytes_to_offset(s, pa - base, t) >> 12);
+      foreign = seg_home(st) != my_p;
+    }
+  }
+  // each wave orders its own payload stores before the block releases: at
+  // agent scope if one of its slices lives in another XCD's segment
+  if (!(s->flags & kFlagNoPayloadFence)) {
+    if (s->num_parts == 1 || __ballot(foreign))
+      utils::fence_release();
+    else
+      detail::release_to_xcd();
+  }
+  bool grouped = false;
+  if (slice) {
+    uint32_t h = (blk * 0x9E3779B1u) >> 21;  // 11 bits
+    for (int probe = 0; probe < 64; ++probe) {
+      const uint32_t prev = atomicCAS(&L.key[h], 0u, blk + 1);
+      if (prev == 0u || prev == blk + 1) {
+        atomicAdd(&L.cnt[h], 1u);
+        grouped = true;
+        break;
+      }
+      h = (h + 1) & (kFreeSlots - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kFreeSlots; i += blockDim.x) {
+    const uint32_t key = L.key[i];
+    if (!key) continue;
+    const uint32_t b = key - 1, k = L.cnt[i];
+    L.key[i] = 0;
+    L.cnt[i] = 0;
+    const uint32_t old = utils::atomic_add(&detail::block_at(s, b)->free_counter, k);
+    if (old + k == kSlicesPerBlock) {
+      utils::atomic_exch(&detail::block_at(s, b)->free_counter, 0u);
+      detail::free_block(s, b, my_p);
+    }
+  }
+  // multi-segment runs, invalid pointers and table overflow: wave path
+  const bool rest = ptr != nullptr && !grouped;
+  if (__ballot(rest)) {
+    if (rest) detail::free_impl(s, ptr);
+  }
+  __syncthreads();
+}
+
+}  // namespace batch
+}  // namespace allocators
+}  // namespace gallatin
+
+#endif

@@ -183,6 +183,16 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
                   uint32_t rounds, uint64_t *counters_dev,
                   uint32_t block_threads, void *stream);
 
+/* The same churn through the workgroup-batched path (batched.hpp): one
+ * persistent launch of `workgroups` workgroups (0 = fill the device) of
+ * `threads` threads (256, 512 or 1024; 0 = 256) walks chunks of `threads`
+ * stream threads; each round a workgroup frees its replaced allocations as
+ * one batch and allocates as one batch.  No reference counterpart (the
+ * reference serves warps, gallatin.cuh:865). */
+int gal_run_churn_batched(gal_allocator *a, uint64_t seed, uint64_t nthreads,
+                          uint32_t rounds, uint64_t *counters_dev,
+                          uint32_t workgroups, uint32_t threads, void *stream);
+
 /* ---- measurement kernels (ceilings for the roofline) ---- */
 /* n_ops relaxed agent-scope atomicAdd(u32) with the returned value kept:
  * op i targets word ((i / lanes_per_address) * stride) % span, so

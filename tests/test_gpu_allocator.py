@@ -185,6 +185,25 @@ def test_churn_patterns_and_counts(gpu):
     a.free_on_device()
 
 
+@pytest.mark.parametrize("partitions", [0, 1])
+def test_churn_batched_patterns_and_counts(gpu, partitions):
+    """The workgroup-batched path serves the same stream: every pair
+    completes, every pattern reads back, the pool drains to fresh."""
+    import torch
+    n, rounds, seed = 1 << 20, 10, 10
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(
+        8 << 30, 7, print_info=False, slots_min=32, partitions=partitions)
+    ctr = a.run_churn_batched(seed, n, rounds)
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    assert c[gpu.CTR_BAD] == 0
+    assert c[gpu.CTR_MISSES] == 0
+    assert c[gpu.CTR_PAIRS] == H.expected_churn_pairs(seed, n, rounds)
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_global_allocator(gpu):
     import torch
     gpu.init_global_allocator(1 << 30, 42, print_info=False)
