@@ -1,0 +1,115 @@
+"""Summaries of rocprofv3 output for the kernels of this library.
+
+    python -m gallatin_amd.profiling stats <kernel_stats.csv> [--title T] [--command C]
+        Markdown table of per-kernel launch counts and durations.
+
+    python -m gallatin_amd.profiling traffic <fetch counter_collection.csv>
+        <write counter_collection.csv> --copy-bytes B --workload W
+        [--match key=int,...] [--out F]
+        HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run.
+        Each counter is scaled by what it reads for k_bench_copy moving B
+        bytes (B read and B written) in the same run, so the result is in
+        bytes actually moved, whatever unit the counter definition assumes.
+"""
+import argparse
+import collections
+import csv
+import json
+import sys
+
+
+def kernel_key(name):
+    """`void k_churn_batched<256>(gal_state*, ...)` -> `k_churn_batched<256>`."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def stats_markdown(path, title=None, command=None):
+    rows = list(csv.DictReader(open(path)))
+    out = []
+    if title:
+        out.append(f"# {title}\n")
+    if command:
+        out.append("Command:\n\n```\n" + command + "\n```\n")
+    out.append("| kernel | calls | avg ms | min ms | max ms | share |")
+    out.append("|---|---|---|---|---|---|")
+    for r in rows:
+        out.append(f"| `{kernel_key(r['Name'])}` | {r['Calls']} | "
+                   f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['MinNs']) / 1e6:.3f} | "
+                   f"{float(r['MaxNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} % |")
+    return "\n".join(out) + "\n"
+
+
+def _per_kernel(path, counter):
+    vals = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            vals[kernel_key(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return vals
+
+
+def traffic(fetch_csv, write_csv, copy_bytes, workload, match=None):
+    fetch = _per_kernel(fetch_csv, "FETCH_SIZE")
+    write = _per_kernel(write_csv, "WRITE_SIZE")
+    if "k_bench_copy" not in fetch or "k_bench_copy" not in write:
+        raise SystemExit("both runs must include k_bench_copy for the scale factors")
+    # the first copy launch is a warm-up that may fault pages in; use the median
+    fc = sorted(fetch["k_bench_copy"])[len(fetch["k_bench_copy"]) // 2]
+    wc = sorted(write["k_bench_copy"])[len(write["k_bench_copy"]) // 2]
+    f_scale = copy_bytes / (fc * 1024.0)
+    w_scale = copy_bytes / (wc * 1024.0)
+    out = {"workload": workload, "match": match or {}, "copy_bytes": copy_bytes,
+           "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for k in sorted(set(fetch) & set(write)):
+        if k.startswith("__amd") or k.startswith("at::") or k.startswith("void at::"):
+            continue
+        f = sum(fetch[k]) / len(fetch[k])
+        w = sum(write[k]) / len(write[k])
+        out["kernels"][k] = {
+            "dispatches": len(fetch[k]),
+            "fetch_kib": f, "write_kib": w,
+            "bytes": f * 1024.0 * f_scale + w * 1024.0 * w_scale,
+        }
+    return out
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    sub = p.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--title")
+    s.add_argument("--command")
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv")
+    t.add_argument("write_csv")
+    t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--workload", required=True)
+    t.add_argument("--match", default="",
+                   help="key=int,... : the benchmark settings this holds for")
+    t.add_argument("--out")
+    a = p.parse_args(argv)
+    if a.cmd == "stats":
+        sys.stdout.write(stats_markdown(a.csv, a.title, a.command))
+    else:
+        match = dict((kv.split("=")[0], int(kv.split("=")[1]))
+                     for kv in a.match.split(",") if kv)
+        res = traffic(a.fetch_csv, a.write_csv, a.copy_bytes, a.workload, match)
+        text = json.dumps(res, indent=1) + "\n"
+        if a.out:
+            open(a.out, "w").write(text)
+        sys.stdout.write(text)
+
+
+if __name__ == "__main__":
+    main()

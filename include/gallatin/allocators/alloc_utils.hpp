@@ -84,6 +84,11 @@ __device__ __forceinline__ void fence_acq_rel() {
 
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(2); }
 
+// Global thread index of a 1-D launch (alloc_utils.cuh:400-402).
+__device__ __forceinline__ uint64_t get_tid() {
+  return (uint64_t)threadIdx.x + (uint64_t)blockIdx.x * (uint64_t)blockDim.x;
+}
+
 }  // namespace utils
 
 // ------------------------------------------------------------ wave64 ops
