@@ -48,18 +48,35 @@ __device__ inline void add_counters(uint64_t* ctr, const lane_counts& c) {
 // buffer (lane order), and lane r handles chunks r, r + 64, r + 128, ...
 // A chunk's owner is the last lane whose chunk prefix is <= the chunk index,
 // found by a 6-step binary search over the prefixes (one per lane, read with
-// ds_bpermute).  Every lane issues its loads back to back, so a verify costs
-// about one memory round trip instead of one per buffer.  Needs all 64 lanes
-// of the wave active.
+// ds_bpermute).  Needs all 64 lanes of the wave active; the chunk loops are
+// wave-uniform (every lane takes part in every ds_bpermute, a disabled source
+// lane would read as 0) and only the memory access itself is predicated.
+//
+// A buffer of w words (w = ceil(size / 8)) lives in a slice whose size is a
+// multiple of 16 bytes and at least 8 * w, so for odd w the slice extends at
+// least 8 bytes past the last word: every chunk is accessed as one aligned
+// 16-byte vector, and the verify ignores the upper half of a buffer's last
+// chunk when w is odd.  Accesses go through global-address-space pointers
+// (global_load / global_store), so stores never hold up the LDS waits of the
+// owner search, and the verify keeps kVerifyDepth loads per lane in flight.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) u64x2 global_u64x2;
+
+constexpr int kVerifyDepth = 4;
+
 struct chunk_plan {
-  uint32_t prefix;  // chunks owned by lower lanes
-  uint32_t total;   // chunks of the whole wave
+  uint32_t prefix;    // chunks owned by lower lanes
+  uint32_t total;     // chunks of the whole wave
+  uint32_t full_end;  // chunks below this index of this lane's buffer are full
+  uint64_t adjusted;  // buffer address - 16 * prefix
 };
 
-__device__ inline chunk_plan plan_chunks(bool on, uint64_t words) {
+__device__ inline chunk_plan plan_chunks(bool on, const void* p, uint64_t words) {
   const uint32_t n = on ? (uint32_t)((words + 1) / 2) : 0u;
   chunk_plan c;
   wave::group_scan(n, ~0ULL, c.prefix, c.total);
+  c.full_end = c.prefix + (on ? (uint32_t)(words / 2) : 0u);
+  c.adjusted = (uint64_t)p - 16ULL * c.prefix;
   return c;
 }
 
@@ -72,51 +89,44 @@ __device__ __forceinline__ int chunk_owner(uint32_t prefix, uint32_t c) {
   return l;
 }
 
-// The chunk loops are wave-uniform: every lane runs every pass and takes
-// part in the ds_bpermute reads (a disabled source lane would read as 0);
-// only the memory access itself is predicated on the chunk existing.
 __device__ inline void wave_fill(bool on, uint64_t* p, uint64_t words, uint64_t pat) {
-  const chunk_plan cp = plan_chunks(on, words);
+  const chunk_plan cp = plan_chunks(on, p, words);
   const uint32_t lane = wave::lane();
   for (uint32_t base = 0; base < cp.total; base += 64) {
     const uint32_t c = base + lane;
     const bool in = c < cp.total;
     const int l = chunk_owner(cp.prefix, in ? c : cp.total - 1);
-    uint64_t* q = reinterpret_cast<uint64_t*>(wave::fetch((uint64_t)p, l));
-    const uint64_t n = wave::fetch(words, l);
+    const uint64_t a = wave::fetch(cp.adjusted, l) + 16ULL * c;
     const uint64_t v = wave::fetch(pat, l);
-    const uint64_t i = 2ULL * ((uint64_t)c - wave::fetch(cp.prefix, l));
-    if (in) {
-      if (i + 1 < n)
-        *reinterpret_cast<ulonglong2*>(q + i) = make_ulonglong2(v, v);
-      else
-        q[i] = v;
-    }
+    if (in) *reinterpret_cast<global_u64x2*>(a) = u64x2{v, v};
   }
 }
 
 // Returns this lane's share of the mismatching words.
 __device__ inline uint32_t wave_verify(bool on, const uint64_t* p, uint64_t words,
                                        uint64_t pat) {
-  const chunk_plan cp = plan_chunks(on, words);
+  const chunk_plan cp = plan_chunks(on, p, words);
   const uint32_t lane = wave::lane();
   uint32_t bad = 0;
-  for (uint32_t base = 0; base < cp.total; base += 64) {
-    const uint32_t c = base + lane;
-    const bool in = c < cp.total;
-    const int l = chunk_owner(cp.prefix, in ? c : cp.total - 1);
-    const uint64_t* q = reinterpret_cast<const uint64_t*>(wave::fetch((uint64_t)p, l));
-    const uint64_t n = wave::fetch(words, l);
-    const uint64_t v = wave::fetch(pat, l);
-    const uint64_t i = 2ULL * ((uint64_t)c - wave::fetch(cp.prefix, l));
-    if (in) {
-      if (i + 1 < n) {
-        const ulonglong2 x = *reinterpret_cast<const ulonglong2*>(q + i);
-        bad += (x.x != v) + (x.y != v);
-      } else {
-        bad += q[i] != v;
-      }
+  for (uint32_t base = 0; base < cp.total; base += 64 * kVerifyDepth) {
+    uint64_t a[kVerifyDepth], v[kVerifyDepth];
+    bool in[kVerifyDepth], full[kVerifyDepth];
+#pragma unroll
+    for (int k = 0; k < kVerifyDepth; ++k) {
+      const uint32_t c = base + 64u * k + lane;
+      in[k] = c < cp.total;
+      const int l = chunk_owner(cp.prefix, in[k] ? c : cp.total - 1);
+      a[k] = wave::fetch(cp.adjusted, l) + 16ULL * c;
+      v[k] = wave::fetch(pat, l);
+      full[k] = c < wave::fetch(cp.full_end, l);
     }
+    u64x2 x[kVerifyDepth];
+#pragma unroll
+    for (int k = 0; k < kVerifyDepth; ++k)
+      if (in[k]) x[k] = *reinterpret_cast<const global_u64x2*>(a[k]);
+#pragma unroll
+    for (int k = 0; k < kVerifyDepth; ++k)
+      if (in[k]) bad += (x[k].x != v[k]) + (full[k] && x[k].y != v[k]);
   }
   return bad;
 }
@@ -224,7 +234,7 @@ __global__ void k_churn(gal_state* s, uint64_t seed, uint64_t n, uint32_t rounds
 
 // The same churn through the workgroup-batched path: a persistent grid walks
 // chunks of blockDim stream threads; for every round the workgroup frees its
-// replaced allocations as one batch and allocates the new ones as one batch.
+// replaced allocations and allocates the new ones as one batch step.
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) void k_churn_batched(gal_state* s, uint64_t seed,
                                                             uint64_t n, uint32_t rounds,
@@ -244,15 +254,18 @@ __global__ __launch_bounds__(kThreads) void k_churn_batched(gal_state* s, uint64
           active && (r == 0 || r == rounds || gal_churn_coin(seed, tid, r) != 0);
       const bool do_free = replace && live != nullptr;
       c.bad += wave_verify(do_free, live, live_words, gal_churn_pattern(tid, live_round));
-      batch::batch_free(s, do_free ? live : nullptr, lds);
+      void* old = do_free ? live : nullptr;
       if (do_free) {
         c.pairs += 1;
         c.bytes += (uint64_t)live_words * 8;
         live = nullptr;
       }
-      if (r == rounds) break;
+      if (r == rounds) {
+        batch::batch_free(s, old, lds);
+        break;
+      }
       const uint32_t size = (uint32_t)gal_churn_size(seed, tid, r);
-      uint64_t* np = static_cast<uint64_t*>(batch::batch_malloc(s, replace, size, lds));
+      uint64_t* np = static_cast<uint64_t*>(batch::batch_step(s, old, replace, size, lds));
       if (replace) {
         c.mallocs += 1;
         if (!np) c.misses += 1;

@@ -1,16 +1,19 @@
 // Workgroup-batched malloc / free: the requests of a whole workgroup are
 // bucketed in LDS and served together.
 //
-//   batch_malloc: requests are counted per size class in LDS; one wave then
-//     claims every class's whole count at once, lane c for class c, with one
+//   malloc: requests are counted per size class in LDS; one wave then claims
+//     every class's whole count at once, lane c for class c, with one
 //     counter atomic per class (more only when a claim runs past slice 4095
 //     and the block is replaced), and each request takes its slice by its
 //     rank in the class.  This is the reference's coalesced group claim
 //     (gallatin.cuh:865-1078, block.cuh:245-264) with the group widened from
 //     a warp to a workgroup.
-//   batch_free: requests are grouped per block in an LDS hash table and each
-//     block gets one free-counter atomic (gallatin.cuh:1774-1827 with a
+//   free: requests are grouped per block in an LDS hash table and each block
+//     gets one free-counter atomic (gallatin.cuh:1774-1827 with a
 //     workgroup-wide team).
+//   batch_step does one free and one malloc per thread with two barriers,
+//     the class claims of wave 0 running beside the block releases of the
+//     other waves.
 //
 // Requests the batch cannot serve (whole blocks, multi-segment runs, runs of
 // 4 KiB slices, deterministic mode, or a class that needs more than kRanges
@@ -98,36 +101,117 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
   if (c < T) L.nranges[c] = nr;
 }
 
+// ---- malloc side ----
+
+struct malloc_req {
+  uint32_t t;     // size class
+  uint32_t rank;  // position among the workgroup's requests of class t
+  bool batched;   // served by the class claims (else the wave path)
+};
+
+// Phase 1: class and rank of this thread's request.
+__device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint64_t size,
+                                                   lds_state& L) {
+  malloc_req r{0u, 0u, false};
+  if (want && !detail::deterministic(s)) {
+    r.t = size < s->smallest
+              ? 0u
+              : (uint32_t)(utils::get_first_bit_bigger(size) - (int)s->smallest_bits);
+    r.batched = r.t < s->num_trees;
+  }
+  if (r.batched) r.rank = atomicAdd(&L.count[r.t], 1u);
+  return r;
+}
+
+// Phase 3: the byte offset of this thread's slice, or kFail.  The ranges
+// are rewritten only by the next batch's claims, after a barrier every
+// thread reaches after reading them here.
+__device__ __forceinline__ uint64_t malloc_take(gal_state* s, const malloc_req& r,
+                                                const lds_state& L) {
+  if (!r.batched) return kFail;
+  for (uint32_t k = 0; k < L.nranges[r.t]; ++k) {
+    const uint32_t f = L.first[r.t][k];
+    if (r.rank >= f && r.rank < f + L.len[r.t][k]) {
+      const uint64_t off = (uint64_t)L.start[r.t][k] + (r.rank - f) +
+                           ((uint64_t)L.blk[r.t][k] << 12);
+      return detail::offset_to_bytes(s, off, r.t);
+    }
+  }
+  return kFail;
+}
+
+// ---- free side ----
+
+// Phase 1: orders this wave's payload before the block releases (agent
+// scope if one of its slices lives in another XCD's segment) and groups the
+// slice frees per block in the LDS table.  Returns false for a pointer the
+// table does not take (multi-segment run, invalid pointer, table overflow),
+// which then goes through the wave path.
+__device__ __forceinline__ bool free_stage(gal_state* s, void* ptr, lds_state& L,
+                                           uint32_t my_p) {
+  const uint64_t base = (uint64_t)s->memory;
+  const uint64_t pa = (uint64_t)ptr;
+  bool slice = false, foreign = false;
+  uint32_t blk = 0;
+  if (ptr != nullptr && !detail::deterministic(s) &&
+      (pa - base) < (s->num_segments << s->seg_shift)) {
+    const uint64_t seg = (pa - base) >> s->seg_shift;
+    const uint64_t st = utils::atomic_load(&s->seg_state[seg]);
+    const uint32_t t = (uint32_t)(st & 0xFFFF);
+    if (t < s->num_trees) {
+      slice = true;
+      blk = (uint32_t)(detail::bytes_to_offset(s, pa - base, t) >> 12);
+      foreign = seg_home(st) != my_p;
+    }
+  }
+  if (!(s->flags & kFlagNoPayloadFence) && __ballot(ptr != nullptr)) {
+    if (s->num_parts == 1 || __ballot(foreign))
+      utils::fence_release();
+    else
+      detail::release_to_xcd();
+  }
+  if (!slice) return false;
+  uint32_t h = (blk * 0x9E3779B1u) >> 21;  // 11 bits
+  for (int probe = 0; probe < 64; ++probe) {
+    const uint32_t prev = atomicCAS(&L.key[h], 0u, blk + 1);
+    if (prev == 0u || prev == blk + 1) {
+      atomicAdd(&L.cnt[h], 1u);
+      return true;
+    }
+    h = (h + 1) & (kFreeSlots - 1);
+  }
+  return false;
+}
+
+// Phase 2: one free-counter atomic per block in the table, by threads
+// [first, blockDim) of the workgroup; empties the table.
+__device__ __forceinline__ void free_scan(gal_state* s, lds_state& L, uint32_t my_p,
+                                          uint32_t first) {
+  const uint32_t span = blockDim.x - first;
+  for (uint32_t i = threadIdx.x - first; i < (uint32_t)kFreeSlots; i += span) {
+    const uint32_t key = L.key[i];
+    if (!key) continue;
+    const uint32_t b = key - 1, k = L.cnt[i];
+    L.key[i] = 0;
+    L.cnt[i] = 0;
+    const uint32_t old = utils::atomic_add(&detail::block_at(s, b)->free_counter, k);
+    if (old + k == kSlicesPerBlock) {
+      utils::atomic_exch(&detail::block_at(s, b)->free_counter, 0u);
+      detail::free_block(s, b, my_p);
+    }
+  }
+}
+
+// ---- workgroup entry points ----
+
 // Each thread with `want` gets a pointer of `size` bytes (or nullptr).
 __device__ inline void* batch_malloc(gal_state* s, bool want, uint64_t size, lds_state& L) {
-  const uint32_t T = s->num_trees;
   const uint32_t p = detail::wave_part(s);
-  uint32_t t = 0;
-  bool batched = false;
-  if (want && !detail::deterministic(s)) {
-    t = size < s->smallest ? 0u
-                           : (uint32_t)(utils::get_first_bit_bigger(size) - (int)s->smallest_bits);
-    batched = t < T;
-  }
-  uint32_t rank = 0;
-  if (batched) rank = atomicAdd(&L.count[t], 1u);
+  const malloc_req r = malloc_stage(s, want, size, L);
   __syncthreads();
   if (threadIdx.x < 64) claim_classes(s, L, p);
   __syncthreads();
-  uint64_t bytes = kFail;
-  if (batched) {
-    for (uint32_t r = 0; r < L.nranges[t]; ++r) {
-      if (rank >= L.first[t][r] && rank < L.first[t][r] + L.len[t][r]) {
-        const uint64_t off = (uint64_t)L.start[t][r] + (rank - L.first[t][r]) +
-                             ((uint64_t)L.blk[t][r] << 12);
-        bytes = detail::offset_to_bytes(s, off, t);
-        break;
-      }
-    }
-  }
-  // (the ranges are rewritten only after the next batch's first barrier,
-  // which every thread reaches after reading them here)
-  // everything else through the wave path
+  const uint64_t bytes = malloc_take(s, r, L);
   void* out = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
   const bool rest = want && bytes == kFail;
   if (__ballot(rest)) {
@@ -141,63 +225,46 @@ __device__ inline void* batch_malloc(gal_state* s, bool want, uint64_t size, lds
 
 // Each thread frees `ptr` (nullptr: nothing).
 __device__ inline void batch_free(gal_state* s, void* ptr, lds_state& L) {
-  const uint32_t T = s->num_trees;
   const uint32_t my_p = detail::wave_part(s);
-  const uint64_t base = (uint64_t)s->memory;
-  const uint64_t pa = (uint64_t)ptr;
-  bool slice = false, foreign = false;
-  uint32_t blk = 0;
-  if (ptr != nullptr && !detail::deterministic(s) &&
-      (pa - base) < (s->num_segments << s->seg_shift)) {
-    const uint64_t seg = (pa - base) >> s->seg_shift;
-    const uint64_t st = utils::atomic_load(&s->seg_state[seg]);
-    const uint32_t t = (uint32_t)(st & 0xFFFF);
-    if (t < T) {
-      slice = true;
-      blk = (uint32_t)(detail::bytes_to_offset(s, pa - base, t) >> 12);
-      foreign = seg_home(st) != my_p;
-    }
-  }
-  // each wave orders its own payload stores before the block releases: at
-  // agent scope if one of its slices lives in another XCD's segment
-  if (!(s->flags & kFlagNoPayloadFence)) {
-    if (s->num_parts == 1 || __ballot(foreign))
-      utils::fence_release();
-    else
-      detail::release_to_xcd();
-  }
-  bool grouped = false;
-  if (slice) {
-    uint32_t h = (blk * 0x9E3779B1u) >> 21;  // 11 bits
-    for (int probe = 0; probe < 64; ++probe) {
-      const uint32_t prev = atomicCAS(&L.key[h], 0u, blk + 1);
-      if (prev == 0u || prev == blk + 1) {
-        atomicAdd(&L.cnt[h], 1u);
-        grouped = true;
-        break;
-      }
-      h = (h + 1) & (kFreeSlots - 1);
-    }
-  }
+  const bool grouped = free_stage(s, ptr, L, my_p);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < (uint32_t)kFreeSlots; i += blockDim.x) {
-    const uint32_t key = L.key[i];
-    if (!key) continue;
-    const uint32_t b = key - 1, k = L.cnt[i];
-    L.key[i] = 0;
-    L.cnt[i] = 0;
-    const uint32_t old = utils::atomic_add(&detail::block_at(s, b)->free_counter, k);
-    if (old + k == kSlicesPerBlock) {
-      utils::atomic_exch(&detail::block_at(s, b)->free_counter, 0u);
-      detail::free_block(s, b, my_p);
-    }
-  }
-  // multi-segment runs, invalid pointers and table overflow: wave path
+  free_scan(s, L, my_p, 0);
   const bool rest = ptr != nullptr && !grouped;
   if (__ballot(rest)) {
     if (rest) detail::free_impl(s, ptr);
   }
   __syncthreads();
+}
+
+// One step of a workgroup: every thread frees `old_ptr` (nullptr: nothing)
+// and, with `want`, gets a pointer of `size` bytes (or nullptr), with two
+// barriers in all.  Wave 0 claims the size classes while the other waves
+// release the freed blocks.  The new slices never come from the frees of
+// the same step (a block returns to service only once all its slices are
+// free), so doing both at once changes no result of either.
+__device__ inline void* batch_step(gal_state* s, void* old_ptr, bool want, uint64_t size,
+                                   lds_state& L) {
+  const uint32_t p = detail::wave_part(s);
+  const bool grouped = free_stage(s, old_ptr, L, p);
+  const malloc_req r = malloc_stage(s, want, size, L);
+  __syncthreads();
+  if (threadIdx.x < 64) claim_classes(s, L, p);
+  free_scan(s, L, p, blockDim.x > 64 ? 64u : 0u);
+  __syncthreads();
+  const bool rest_free = old_ptr != nullptr && !grouped;
+  if (__ballot(rest_free)) {
+    if (rest_free) detail::free_impl(s, old_ptr);
+  }
+  const uint64_t bytes = malloc_take(s, r, L);
+  void* out = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
+  const bool rest = want && bytes == kFail;
+  if (__ballot(rest)) {
+    void* q = nullptr;
+    if (rest) q = detail::malloc_impl(s, size);
+    if (rest) out = q;
+  }
+  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
+  return out;
 }
 
 }  // namespace batch
