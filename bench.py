@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--flags", type=int, default=0)
     p.add_argument("--block-threads", type=int, default=256)
     p.add_argument("--workgroups", type=int, default=0)
-    p.add_argument("--batch-threads", type=int, default=256)
+    p.add_argument("--batch-threads", type=int, default=1024)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-ceilings", action="store_true")
     p.add_argument("--cpu-sample-threads", type=int, default=1 << 22)

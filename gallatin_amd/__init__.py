@@ -42,7 +42,7 @@ CAPI_SYMBOLS = [
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
     "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
-    "gal_last_error",
+    "gal_last_error", "gal_get_stats",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
         L.gal_from_device_state.argtypes = [_p]
         L.gal_print_info.argtypes = [_p]
         L.gal_get_info.argtypes = [_p, ctypes.POINTER(Info)]
+        L.gal_get_stats.argtypes = [_p, _p, _u32]
         L.gal_drain.argtypes = [_p, _p]
         L.gal_reboot.argtypes = [_p, _p]
         L.gal_snapshot.argtypes = [_p, _p, _p, _p]
@@ -196,6 +197,17 @@ class Gallatin:
     def print_info(self):
         _check(lib().gal_print_info(self.handle))
 
+    STAT_NAMES = ("errors", "boot_failed", "segments_set_up", "segments_released",
+                  "blocks_returned", "refills", "err_chunk_id", "err_ring_order",
+                  "err_block_free", "err_ring_empty", "err_setup", "err_tag",
+                  "err_multi_head", "err_invalid_free", "miss_no_block", "miss_retries")
+
+    def stats(self):
+        """Event counters (gal_get_stats) by name."""
+        buf = (ctypes.c_uint64 * 16)()
+        _check(lib().gal_get_stats(self.handle, buf, 16))
+        return {n: int(buf[i]) for i, n in enumerate(self.STAT_NAMES)}
+
     def info(self):
         i = Info()
         _check(lib().gal_get_info(self.handle, ctypes.byref(i)))
@@ -274,7 +286,7 @@ class Gallatin:
         return ctr
 
     def run_churn_batched(self, seed, nthreads, rounds, counters=None, workgroups=0,
-                          threads=256):
+                          threads=0):
         ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
         _check(lib().gal_run_churn_batched(self.handle, seed, nthreads, rounds,
                                            ctr.data_ptr(), workgroups, threads, _stream()))

@@ -102,7 +102,7 @@ __global__ void k_drain(gal_state* s) {
     for (uint32_t i = 0; i < s->slot_count[t]; ++i) {
       if (wave::is_first()) {
         const uint32_t blk = utils::atomic_exch(&s->slots[s->slot_off[t] + i], kNullBlock);
-        if (blk != kNullBlock) {
+        if (blk < kRefilling) {
           uint32_t c = utils::atomic_load(&detail::block_at(s, blk)->malloc_counter) & kCountMask;
           if (c > 4096u) c = 4096u;
           // act as the block's home so a completed segment is released
@@ -492,6 +492,15 @@ int gal_get_info(gal_allocator* a, gal_info* out) {
   out->errors = stats[kStatErrors];
   out->overhead_bytes = a->meta_bytes;
   return GAL_OK;
+}
+
+int gal_get_stats(gal_allocator* a, uint64_t* out, uint32_t n) {
+  if (!a || (!out && n)) return fail(GAL_ERR_INVALID, "null argument");
+  unsigned long long stats[kStatCount];
+  const hipError_t e = hipMemcpy(stats, a->host.stats, sizeof(stats), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "stats copy");
+  for (uint32_t i = 0; i < n && i < (uint32_t)kStatCount; ++i) out[i] = stats[i];
+  return (int)kStatCount;
 }
 
 int gal_print_info(gal_allocator* a) {

@@ -62,7 +62,7 @@ __device__ inline void add_counters(uint64_t* ctr, const lane_counts& c) {
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) u64x2 global_u64x2;
 
-constexpr int kVerifyDepth = 4;
+constexpr int kVerifyDepth = 8;
 
 struct chunk_plan {
   uint32_t prefix;    // chunks owned by lower lanes
@@ -383,9 +383,9 @@ int gal_run_churn_batched(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t 
   if (n == 0) return GAL_OK;
   hipStream_t st = (hipStream_t)stream;
   switch (threads) {
-    case 0:
     case 256: return launch_churn_batched<256>(a, seed, n, rounds, ctr, workgroups, st);
     case 512: return launch_churn_batched<512>(a, seed, n, rounds, ctr, workgroups, st);
+    case 0:
     case 1024: return launch_churn_batched<1024>(a, seed, n, rounds, ctr, workgroups, st);
     default: return fail(GAL_ERR_INVALID, "batched churn runs 256, 512 or 1024 threads per workgroup");
   }

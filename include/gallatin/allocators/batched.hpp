@@ -69,8 +69,17 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
   while (__ballot(open)) {
     uint64_t sb = kNullBlock;
     if (open) sb = detail::pinned_block(s, c, p);
+    // classes that find no pinned block refill their start slot, in lane
+    // (class) order
+    for (uint64_t m = __ballot(open && (uint32_t)sb == kNullBlock); m; m &= m - 1) {
+      const int l = wave::first(m);
+      const uint32_t sl = (uint32_t)(wave::readlane(sb, l) >> 32);
+      const uint32_t got = detail::refill_slot(s, (uint32_t)l, p, sl, false);
+      if ((int)c == l) sb = ((uint64_t)sl << 32) | got;
+    }
     const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
-    const bool have = open && blk != kNullBlock;
+    const bool have = open && blk < kRefilling;
+    const bool waiting = open && blk == kRefilling;  // another wave refills
     uint32_t old = 0;
     if (have) old = utils::atomic_add(&detail::block_at(s, blk)->malloc_counter, need - served);
     const uint32_t cnt = old & kCountMask;
@@ -87,14 +96,19 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
       ++nr;
       served += take;
     }
-    // the claim that took slice 4095 replaces the block, one class at a time
-    const uint64_t rep = __ballot(got && cnt + take == 4096u);
+    // the claim that took slice 4095 replaces the block, one class at a
+    // time; so does a class that keeps finding its block full (its 4095
+    // claimer may have detached nothing); the slot CAS admits one of them
+    const uint64_t rep = __ballot((got && cnt + take == 4096u) ||
+                                  (tag_ok && cnt >= 4096u && attempts >= kHelpAfter));
     for (uint64_t m = rep; m; m &= m - 1) {
       const int l = wave::first(m);
       detail::replace_block(s, wave::readlane(c, l), p, wave::readlane(slot, l),
                             wave::readlane(blk, l), false);
     }
-    if (open && !(served == need || nr == (uint32_t)kRanges || ++attempts >= s->lim_slice))
+    if (__ballot(waiting)) utils::backoff();
+    if (open && (have || waiting) &&
+        !(served == need || nr == (uint32_t)kRanges || ++attempts >= s->lim_slice))
       continue;
     open = false;
   }

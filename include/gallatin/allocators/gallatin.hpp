@@ -100,6 +100,10 @@ __device__ __forceinline__ gal_block* block_at(const gal_state* s, uint32_t blk)
 __device__ __forceinline__ void add_stat(gal_state* s, int idx, uint64_t v) {
   atomic_add(&s->stats[idx], (unsigned long long)v);
 }
+__device__ __forceinline__ void add_error(gal_state* s, int kind, uint64_t v = 1) {
+  add_stat(s, kind, v);
+  add_stat(s, kStatErrors, v);
+}
 
 // Stores of this wave have reached the L2 of its XCD.
 __device__ __forceinline__ void release_to_xcd() {
@@ -116,7 +120,7 @@ __device__ inline void clear_chunk_id(gal_state* s, uint64_t seg) {
     if (seen == cur) return;
     cur = seen;
   }
-  add_stat(s, kStatErrors, 1);
+  add_error(s, kStatErrChunkId);
 }
 
 // ----------------------------------------------------------- per lane ops
@@ -135,7 +139,7 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
   // finalize_free_queue: tickets complete in order
   for (uint32_t spin = 0; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
     if (spin >= kSpinCap) {
-      add_stat(s, kStatErrors, 1);
+      add_error(s, kStatErrRingOrder);
       break;
     }
     utils::backoff();
@@ -172,7 +176,7 @@ __device__ __noinline__ void free_block(gal_state* s, uint32_t blk, uint32_t my_
   const uint64_t st = atomic_load(&s->seg_state[seg]);
   const uint32_t t = (uint32_t)(st & 0xFFFF);
   if (t >= s->num_trees) {
-    add_stat(s, kStatErrors, 1);
+    add_error(s, kStatErrBlockFree);
     return;
   }
   return_block(s, blk, seg, t, seg_home(st), my_p);
@@ -187,6 +191,20 @@ __device__ __forceinline__ void block_release(gal_state* s, uint32_t blk, uint32
     atomic_exch(&block_at(s, blk)->free_counter, 0u);
     free_block(s, blk, my_p);
   }
+}
+
+// Give back a block that was handed out but never pinned (it lost a slot
+// race).  It is retired the way a full block is: this lane claims every
+// slice not yet claimed and credits them as freed.  A wave that read the
+// block's id from a slot long ago may still add to its counter; whatever
+// it took is freed by that wave, and the block returns to its segment only
+// after that.  So a block waiting in a segment ring never shows claimable
+// slices (a returned block with count 0 would hand slices of a free segment
+// to such a wave).  Per lane.
+__device__ inline void retire_block(gal_state* s, uint32_t blk, uint32_t my_p) {
+  const uint32_t old = atomic_add(&block_at(s, blk)->malloc_counter, kSlicesPerBlock);
+  const uint32_t c = old & kCountMask;
+  if (c < kSlicesPerBlock) block_release(s, blk, kSlicesPerBlock - c, my_p);
 }
 
 // memory_table.cuh:915-993 get_block for a segment of class t, home p.  The
@@ -209,7 +227,7 @@ __device__ inline uint64_t get_block(gal_state* s, uint64_t seg, uint32_t t, uin
                       kNullBlock);
   }
   if (blk == kNullBlock) {  // memory_table.cuh:977-982 traps here
-    add_stat(s, kStatErrors, 1);
+    add_error(s, kStatErrRingEmpty);
     return kNullBlock;
   }
   // init_malloc (block.cuh:215-224): the tag carries class and home
@@ -227,8 +245,9 @@ __device__ __noinline__ void free_segments(gal_state* s, uint64_t seg, uint64_t 
 }
 
 // shared_block_storage.cuh:119-134 get_valid_block_index + get_my_block for
-// the slots of class t in partition p: from the start slot skip empty slots
-// (at most 30 steps).  Returns (slot << 32) | block.
+// the slots of class t in partition p: from the start slot skip slots
+// without a block (at most 30 steps).  Returns (slot << 32) | block; with no
+// block found, the start slot and its content (kNullBlock or kRefilling).
 __device__ inline uint64_t pinned_block(const gal_state* s, uint32_t t, uint32_t p) {
   const uint32_t per = s->slot_count[t] / s->num_parts;
   const uint32_t base = p * per;
@@ -241,11 +260,16 @@ __device__ inline uint64_t pinned_block(const gal_state* s, uint32_t t, uint32_t
   const int64_t original = rel;
   int64_t i = rel;
   uint32_t blk = atomic_load(&sl[i]);
+  const uint32_t first = blk;
   int counter = 0;
-  while (blk == kNullBlock && i != original - 1) {
+  while (blk >= kRefilling && i != original - 1) {
     i = (i + 1) % per;
     blk = atomic_load(&sl[i]);
     if (++counter >= kSlotScanCutoff) break;
+  }
+  if (blk >= kRefilling) {
+    i = original;
+    blk = first;
   }
   return ((uint64_t)(base + (uint32_t)i) << 32) | blk;
 }
@@ -307,11 +331,13 @@ __device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t, uin
 }
 
 // gallatin.cuh:1465-1510 gather_new_segment: smallest free segment.
+// kNoSegment: the segment tree had none; kFail: set-up failed.
+constexpr uint64_t kNoSegment = kFail - 1;
 __device__ inline uint64_t gather_new_segment(gal_state* s, uint32_t t, uint32_t p) {
   const uint64_t seg = veb::take_first(s->seg_tree);
-  if (seg == kFail) return kFail;
+  if (seg == kFail) return kNoSegment;
   if (!setup_segment(s, seg, t, p)) {
-    if (wave::is_first()) add_stat(s, kStatErrors, 1);
+    if (wave::is_first()) add_error(s, kStatErrSetup);
     return kFail;
   }
   if (wave::is_first()) veb::insert(class_tree(s, t, p), seg);
@@ -333,7 +359,9 @@ __device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bo
   return x;
 }
 
-// gallatin.cuh:1525-1631 request_new_block_from_tree.
+// gallatin.cuh:1525-1631 request_new_block_from_tree.  With no segment in
+// the class tree and none free it polls (up to lim_request times): frees in
+// flight on other waves return segments within microseconds.
 __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t p,
                                              bool det) {
   int attempts = 0;
@@ -348,8 +376,9 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
       }
       seg = gather_new_segment(s, t, p);
       unlock(s, lk);
-      if (seg == kFail) {
+      if (seg == kFail || seg == kNoSegment) {
         ++attempts;
+        if (seg == kNoSegment) utils::backoff();
         continue;
       }
     }
@@ -370,21 +399,46 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
   return kNullBlock;
 }
 
-// gallatin.cuh:1083-1138 replace_block.
+// Refill an empty pinned slot of class t (wave op).  Returns the block now
+// in the slot; kRefilling when another wave is fetching it (try again
+// later); kNullBlock when this wave fetched and none could be had (the slot
+// is left empty for a later try).  One wave at a time fetches for a slot:
+// it marks the slot kRefilling first.  The reference has no refill: a slot
+// stays null once a replacement failed (gallatin.cuh:1091-1116).
+__device__ __noinline__ uint32_t refill_slot(gal_state* s, uint32_t t, uint32_t p,
+                                             uint32_t slot, bool det) {
+  uint32_t* sp = s->slots + s->slot_off[t] + slot;
+  uint32_t cur = kNullBlock;
+  if (wave::is_first()) cur = atomic_cas(sp, kNullBlock, kRefilling);
+  cur = wave::readfirst(cur);
+  if (cur != kNullBlock) return cur;  // a block, or another wave's refill
+  const uint32_t nb = request_new_block(s, t, p, det);
+  if (wave::is_first()) {
+    atomic_exch(sp, nb);  // the block, or empty again
+    if (nb != kNullBlock) add_stat(s, kStatRefills, 1);
+  }
+  return nb;
+}
+
+// gallatin.cuh:1083-1138 replace_block.  The detached slot holds kRefilling
+// until the new block is in, so waiting waves do not fetch blocks of their
+// own; if none can be had the slot is left empty.
 __device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t p,
                                            uint32_t slot, uint32_t blk, bool det) {
   uint32_t* sp = s->slots + s->slot_off[t] + slot;
   uint32_t seen = 0;
-  if (wave::is_first()) seen = atomic_cas(sp, blk, kNullBlock);
+  if (wave::is_first()) {
+    seen = atomic_cas(sp, blk, kRefilling);
+    // the detached block normally has every slice claimed; if a delayed
+    // helper detached a later incarnation that still has free slices,
+    // retire those so the block can complete
+    if (seen == blk &&
+        (atomic_load(&block_at(s, blk)->malloc_counter) & kCountMask) < kSlicesPerBlock)
+      retire_block(s, blk, p);
+  }
   if (wave::readfirst(seen) != blk) return;
   const uint32_t nb = request_new_block(s, t, p, det);
-  if (nb == kNullBlock) return;
-  if (wave::is_first()) {
-    if (atomic_cas(sp, kNullBlock, nb) != kNullBlock) {
-      free_block(s, nb, p);
-      add_stat(s, kStatErrors, 1);
-    }
-  }
+  if (wave::is_first()) atomic_exch(sp, nb);
 }
 
 // Slice phase of malloc: gallatin.cuh:865-1078 malloc_slice_allocation for
@@ -405,9 +459,18 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
     const bool is_leader = pending && leader == (int)lane;
     uint64_t sb = kNullBlock;
     if (is_leader) sb = pinned_block(s, t, p);
+    // a group that finds no pinned block refills its start slot, groups in
+    // lane order of their leaders
+    for (uint64_t m = __ballot(is_leader && (uint32_t)sb == kNullBlock); m; m &= m - 1) {
+      const int l = wave::first(m);
+      const uint32_t sl = (uint32_t)(wave::readlane(sb, l) >> 32);
+      const uint32_t got = refill_slot(s, wave::readlane(t, l), p, sl, det);
+      if ((int)lane == l) sb = ((uint64_t)sl << 32) | got;
+    }
     sb = wave::fetch(sb, leader);
     const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
-    const bool have = pending && blk != kNullBlock;
+    const bool have = pending && blk < kRefilling;
+    const bool waiting = pending && blk == kRefilling;  // another wave refills
     // exclusive prefix in lane order and group total (block.cuh:245-264)
     uint32_t prefix, total;
     if (__ballot(pending && cnt != 1) == 0) {
@@ -443,8 +506,13 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
         }
       }
     }
-    // the group whose claim covers slice 4095 replaces the block
-    const uint64_t rep = __ballot(should_replace);
+    // the group whose claim covers slice 4095 replaces the block; so does a
+    // group that keeps finding its own class's block full (the claimer of
+    // slice 4095 may have reached it through a stale slot read and detached
+    // nothing); the slot CAS lets exactly one of them through
+    const bool full_seen = have && !start_valid && attempts >= kHelpAfter &&
+                           (old >> kTreeShift) == block_tag(t, p);
+    const uint64_t rep = __ballot(should_replace || full_seen);
     for (uint64_t m = __ballot(is_leader && (grp & rep) != 0); m; m &= m - 1) {
       const int l = wave::first(m);
       replace_block(s, wave::readlane(t, l), p, wave::readlane(slot, l),
@@ -462,23 +530,35 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
       const uint32_t b = wave::readlane(blk, wave::first(m));
       if (wave::is_first()) block_release(s, b, 1, p);
     }
-    if (pending && ++attempts >= s->lim_slice) pending = false;
+    // no block to be had: fail now rather than retry; a group waiting for
+    // another wave's refill backs off and tries again
+    const uint64_t nob = __ballot(pending && !have && !waiting);
+    if (pending && !have && !waiting) pending = false;
+    if (__ballot(waiting)) utils::backoff();
+    const bool spent = pending && ++attempts >= s->lim_slice;
+    if (spent) pending = false;
+    const uint64_t spent_m = __ballot(spent);
+    if ((nob | spent_m) && wave::is_first()) {
+      if (nob) add_stat(s, kStatMissNoBlock, wave::count(nob));
+      if (spent_m) add_stat(s, kStatMissRetries, wave::count(spent_m));
+    }
   }
 }
 
 // gallatin.cuh:828-861 malloc_block_allocation + block.cuh:288-340
-// malloc_fill_block, retried as gallatin.cuh:1226-1231.  Byte offset.
+// malloc_fill_block, retried as gallatin.cuh:1226-1231 while blocks are
+// handed out but lost to a concurrent claim.  Byte offset.
 __device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, uint32_t p,
                                                    bool det) {
   for (uint32_t attempt = 0; attempt < s->lim_block; ++attempt) {
     const uint32_t blk = request_new_block(s, bt, p, det);
-    if (blk == kNullBlock) continue;
+    if (blk == kNullBlock) return kFail;
     uint32_t res = 0;
     if (wave::is_first()) {
       const uint32_t merged = atomic_add(&block_at(s, blk)->malloc_counter, 4096u);
       const uint32_t oldc = merged & kCountMask;
       if ((merged >> kTreeShift) != block_tag(bt, p)) {  // check_valid -> trap
-        add_stat(s, kStatErrors, 1);
+        add_error(s, kStatErrTag);
       } else {
         atomic_add(&block_at(s, blk)->free_counter, 4095u - oldc);
         if (oldc != 0)
@@ -506,8 +586,12 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
     }
     if (!got) return kFail;
     const uint64_t idx = veb::gather_multiple(s->seg_tree, n);
+    if (idx == kFail) {  // no free run of n segments now
+      unlock(s, 0);
+      return kFail;
+    }
     uint32_t ok = 0;
-    if (idx != kFail && wave::is_first()) {
+    if (wave::is_first()) {
       // set_tree_id (memory_table.cuh:823-828): head marked T + 1 + n
       uint64_t cur = atomic_load(&s->seg_state[idx]);
       for (int g = 0; g < 4096; ++g) {
@@ -521,7 +605,7 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
         cur = seen;
       }
       if (!ok) {
-        add_stat(s, kStatErrors, 1);
+        add_error(s, kStatErrMultiHead);
         veb::return_multiple(s->seg_tree, idx, n);
       }
     }
@@ -601,7 +685,7 @@ __device__ __forceinline__ void free_impl(gal_state* s, void* ptr) {
   const bool multi = in && t > T && t != kFreeTree;
   const bool slice = in && t < T;
   const uint64_t bad = __ballot(ptr != nullptr && !multi && !slice);
-  if (bad && wave::is_first()) add_stat(s, kStatErrors, wave::count(bad));
+  if (bad && wave::is_first()) add_error(s, kStatErrInvalidFree, wave::count(bad));
   if (!__ballot(multi || slice)) return;
   if (!(s->flags & kFlagNoPayloadFence)) {
     // a free away from the segment's home XCD writes this XCD's L2 back

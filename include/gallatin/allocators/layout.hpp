@@ -29,6 +29,9 @@ constexpr uint32_t kTreeShift = 20;          // block.cuh:53
 constexpr uint32_t kCountMask = (1u << kTreeShift) - 1u;
 constexpr uint32_t kFreeTree = 0xFFFFu;      // memory_table.cuh:485
 constexpr uint32_t kNullBlock = 0xFFFFFFFFu;
+// Pinned-slot marker: a wave is fetching the slot's next block.  Other waves
+// wait for it instead of fetching blocks of their own.
+constexpr uint32_t kRefilling = 0xFFFFFFFEu;
 constexpr uint64_t kFail = ~0ULL;
 constexpr int kMaxLayers = 4;                // universes up to 2^24 segments
 constexpr int kMaxTrees = 24;
@@ -40,6 +43,8 @@ constexpr int kSliceAttempts = 500 * 10;
 constexpr int kBlockAttempts = 500;
 constexpr int kSegmentAttempts = 500;
 constexpr int kSlotScanCutoff = 30;
+// failed claims on a full block before a group helps to replace it
+constexpr uint32_t kHelpAfter = 16;
 // bound on loops the reference leaves unbounded (lost lock races)
 constexpr uint32_t kSpinCap = 1u << 16;
 
@@ -99,14 +104,27 @@ struct gal_seg_queue {
   uint32_t pad;
 };
 
-// stats[] indices
+// stats[] indices.  kStatErrors is the sum of the kinds from kStatErrFirst
+// to kStatErrInvalidFree.
 enum gal_stat {
-  kStatErrors = 0,      // invalid frees and internal inconsistencies
+  kStatErrors = 0,
   kStatBootFailed = 1,
   kStatSegmentsSetUp = 2,
   kStatSegmentsReleased = 3,
   kStatBlocksReturned = 4,
-  kStatCount = 8
+  kStatRefills = 5,         // empty pinned slots given a block again
+  kStatErrFirst = 6,
+  kStatErrChunkId = 6,      // a released segment's chunk id could not be reset
+  kStatErrRingOrder = 7,    // ring completion never came round
+  kStatErrBlockFree = 8,    // block returned into an unformatted segment
+  kStatErrRingEmpty = 9,    // ring entry empty where a block was due
+  kStatErrSetup = 10,       // a free segment could not be formatted
+  kStatErrTag = 11,         // whole-block claim found another class's tag
+  kStatErrMultiHead = 12,   // multi-segment head could not be marked
+  kStatErrInvalidFree = 13, // nullptr, foreign or unformatted-segment frees
+  kStatMissNoBlock = 14,    // slice requests failed: no block to be had
+  kStatMissRetries = 15,    // slice requests failed: retries used up
+  kStatCount = 16
 };
 
 struct gal_state {

@@ -103,6 +103,19 @@ int gal_print_info(gal_allocator *a);
 /* Statistics behind print_info as numbers.  Synchronous. */
 int gal_get_info(gal_allocator *a, gal_info *out);
 
+/* Event counters of the instance (no counterpart in the reference; the
+ * reference prints errors instead).  Copies min(n, GAL_STAT_COUNT) values
+ * and returns GAL_STAT_COUNT.  [0] errors (sum of [6..13]), [1] boot
+ * failed, [2] segments formatted, [3] segments released, [4] blocks
+ * returned, [5] empty pinned slots refilled, [6] chunk id reset
+ * failed, [7] ring completion timed out, [8] block freed into an
+ * unformatted segment, [9] empty ring entry, [10] segment set-up failed,
+ * [11] whole-block tag mismatch, [12] multi-segment head marking failed,
+ * [13] invalid frees, [14] slice requests failed for want of a block, [15]
+ * slice requests failed after all retries.  Synchronous. */
+#define GAL_STAT_COUNT 16
+int gal_get_stats(gal_allocator *a, uint64_t *out, uint32_t n);
+
 /* Quiescent drain (no counterpart in the reference): give back every pinned
  * block's unclaimed slices and release formatted segments whose blocks are
  * all free, so that after all frees the metadata equals a fresh instance.
@@ -185,7 +198,7 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
 
 /* The same churn through the workgroup-batched path (batched.hpp): one
  * persistent launch of `workgroups` workgroups (0 = fill the device) of
- * `threads` threads (256, 512 or 1024; 0 = 256) walks chunks of `threads`
+ * `threads` threads (256, 512 or 1024; 0 = 1024) walks chunks of `threads`
  * stream threads; each round a workgroup frees its replaced allocations as
  * one batch and allocates as one batch.  No reference counterpart (the
  * reference serves warps, gallatin.cuh:865). */

@@ -399,13 +399,14 @@ static uint32_t get_block(orc_alloc *a, uint64_t seg, int t, int *empty) {
   return blk;
 }
 
-/* gallatin.cuh:1465-1510 gather_new_segment. */
+/* gallatin.cuh:1465-1510 gather_new_segment.  -1: no free segment,
+ * -2: the segment could not be set up. */
 static int64_t gather_new_segment(orc_alloc *a, int t) {
   uint64_t seg = orc_veb_malloc_first(a->segment_tree);
   if (seg == ORC_FAIL) return -1;
   if (!setup_segment(a, seg, t)) {
     a->errors++;
-    return -1;
+    return -2;
   }
   orc_veb_insert(a->sub_trees[t], seg);
   return (int64_t)seg;
@@ -413,7 +414,9 @@ static int64_t gather_new_segment(orc_alloc *a, int t) {
 
 /* gallatin.cuh:1525-1631 request_new_block_from_tree, deterministic segment
  * choice (find_first_valid_index).  The tree lock of 1513-1519 is always
- * free in a sequential run. */
+ * free in a sequential run.  With no segment in the class tree and none free
+ * the concurrent version polls for segments freed meanwhile; sequentially
+ * every poll finds the same, so this returns at once. */
 static uint32_t request_new_block(orc_alloc *a, int t) {
   int attempts = 0;
   long spins = 0;
@@ -422,7 +425,8 @@ static uint32_t request_new_block(orc_alloc *a, int t) {
     uint64_t seg = orc_veb_find_first(a->sub_trees[t]);
     if (seg == ORC_FAIL) {
       int64_t s = gather_new_segment(a, t);
-      if (s == -1) {
+      if (s == -1) return NULL_BLOCK;
+      if (s < 0) {
         attempts++;
         continue;
       }
@@ -702,6 +706,22 @@ static void malloc_slices(orc_alloc *a, uint64_t mask, const int *cls,
         slot[j] = valid_slot_index(a, cls[j]);
         blk[j] = a->slots[a->slot_off[cls[j]] + slot[j]];
       }
+    /* F6: a group that finds no pinned block refills the start slot (0 in
+     * this order) in lane order of the leaders; if no block can be had the
+     * group's requests fail at once */
+    for (int j = 0; j < 64; j++)
+      if ((pending >> j & 1) && leader[j] == j && blk[j] == NULL_BLOCK) {
+        uint32_t *sl = a->slots + a->slot_off[cls[j]];
+        uint32_t cur = sl[0];
+        if (cur == NULL_BLOCK) {
+          uint32_t nb = request_new_block(a, cls[j]);
+          if (nb != NULL_BLOCK) sl[0] = nb;
+          cur = nb;
+        }
+        slot[j] = 0;
+        blk[j] = cur;
+        if (cur == NULL_BLOCK) pending &= ~group[j];
+      }
     uint64_t have = 0;
     for (int j = 0; j < 64; j++)
       if (pending >> j & 1) {
@@ -780,7 +800,7 @@ static void malloc_slices(orc_alloc *a, uint64_t mask, const int *cls,
 static uint64_t malloc_block_path(orc_alloc *a, int bt) {
   for (int attempt = 0; attempt < a->lim_block; attempt++) {
     uint32_t blk = request_new_block(a, bt);
-    if (blk == NULL_BLOCK) continue;
+    if (blk == NULL_BLOCK) return ORC_FAIL; /* F6 */
     uint32_t merged = a->blocks[blk].malloc_counter;
     a->blocks[blk].malloc_counter = merged + 4096;
     uint32_t oldc = merged & COUNT_MASK;
@@ -802,7 +822,7 @@ static uint64_t malloc_block_path(orc_alloc *a, int bt) {
 static uint64_t malloc_segment_path(orc_alloc *a, uint32_t n) {
   for (int attempt = 0; attempt < a->lim_segment; attempt++) {
     uint64_t idx = orc_veb_gather_multiple(a->segment_tree, n);
-    if (idx == ORC_FAIL) continue;
+    if (idx == ORC_FAIL) return ORC_FAIL; /* F6 */
     uint32_t id = (uint32_t)a->T + 1 + n;
     if (a->chunk_ids[idx] != FREE_TREE || id >= FREE_TREE) {
       a->errors++;

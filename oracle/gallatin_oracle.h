@@ -36,6 +36,13 @@
  *      so such a block leaks).
  *   F5 free(nullptr), frees outside the pool and frees into an unformatted
  *      segment are counted as errors and ignored (reference: UB / trap).
+ *   F6 a coalesced group that finds no pinned block of its class refills the
+ *      empty start slot with a new block (the reference leaves a slot null
+ *      once a replacement failed, gallatin.cuh:1091-1116, so a class that
+ *      ran out of memory never allocates again); a whole-block request
+ *      fails once no block can be had and a multi-segment request fails
+ *      when no free run of n segments exists, instead of repeating the same
+ *      search 500 times (gallatin.cuh:1226-1251).
  */
 #ifndef GALLATIN_ORACLE_H
 #define GALLATIN_ORACLE_H

@@ -131,6 +131,35 @@ def test_stream_parity(gpu, seed, pool, mix, nsteps):
     a.free_on_device()
 
 
+def test_class_recovers_after_running_out(gpu):
+    """Waves of 64 x 2 KiB until the 256 MiB pool runs out, then every
+    allocation freed, then more 2 KiB waves: the class allocates again
+    (empty pinned slots are refilled) with the oracle's offsets."""
+    n_alloc, n_again = 2200, 8
+    steps = np.zeros(2 * n_alloc + n_again, H.STEP_DTYPE)
+    full = (1 << 64) - 1
+    for i in range(n_alloc):
+        steps[i]["kind"], steps[i]["mask"] = 0, full
+        steps[i]["arg"][:] = 2048
+    for i in range(n_alloc):
+        st = steps[n_alloc + i]
+        st["kind"], st["mask"] = 1, full
+        st["arg"][:] = i * 64 + np.arange(64)
+    for i in range(2 * n_alloc, len(steps)):
+        steps[i]["kind"], steps[i]["mask"] = 0, full
+        steps[i]["arg"][:] = 2048
+    pool = 256 << 20
+    o = oracle.Allocator(pool, limits=QUICK)
+    exp = o.run_stream(steps)
+    assert np.any(exp[:n_alloc * 64] == H.FAIL), "the pool did not run out"
+    assert np.all(exp[2 * n_alloc * 64:] != H.FAIL), "oracle did not recover"
+    a = make(gpu, pool)
+    got = a.run_wave_stream(steps)
+    assert np.array_equal(got, exp)
+    compare_metadata(a, o)
+    a.free_on_device()
+
+
 def test_plain_small_config(gpu):
     """1 GiB pool, 2^20 threads x 10 launches of malloc(16) / write id /
     read back / free; every pair succeeds and the pool returns to fresh."""
@@ -170,6 +199,38 @@ def test_bulk_live_set_disjoint(gpu):
     a.free_on_device()
 
 
+def test_block_and_multi_segment_config(gpu):
+    """64 GiB pool (4096 segments, 3-level trees): 4096 requests uniform in
+    [64 KiB, 64 MiB] (whole blocks and 2-4 segment runs) and 64 requests
+    uniform in [256 MiB, 2 GiB] (16-128 segment runs) in one launch.  Demand
+    is far above the pool, so many return nullptr; the served ones are in
+    the pool, aligned and disjoint and fill most of it.  All are freed in
+    random order, and after a drain every bitmap, chunk id and counter is
+    the fresh state bit for bit."""
+    import torch
+    rng = np.random.default_rng(33)
+    sizes = np.concatenate([rng.integers(64 << 10, (64 << 20) + 1, 4096),
+                            rng.integers(256 << 20, (2 << 30) + 1, 64)]).astype(np.uint64)
+    pool = 64 << 30
+    a = make(gpu, pool, det=False, limits=None)
+    ptrs = a.malloc_bulk(torch.from_numpy(sizes.astype(np.int64)).cuda())
+    torch.cuda.synchronize()
+    p = ptrs.cpu().numpy().view(np.uint64)
+    served = p != 0
+    assert served.sum() > 0
+    H.check_ptrs(p, sizes, a.pool_base, a.num_segments * S)
+    span, _ = H.reserved_vec(sizes)
+    assert int(span[served].sum()) <= pool
+    assert int(span[served].sum()) > pool // 2, "most of the pool should be in use"
+    live = ptrs[torch.from_numpy(np.nonzero(served)[0]).cuda()]
+    perm = torch.from_numpy(rng.permutation(len(live))).cuda()
+    a.free_bulk(live[perm].contiguous())
+    torch.cuda.synchronize()
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_churn_patterns_and_counts(gpu):
     import torch
     n, rounds, seed = 1 << 20, 10, 9
@@ -185,15 +246,15 @@ def test_churn_patterns_and_counts(gpu):
     a.free_on_device()
 
 
-@pytest.mark.parametrize("partitions", [0, 1])
-def test_churn_batched_patterns_and_counts(gpu, partitions):
+@pytest.mark.parametrize("partitions,threads", [(0, 1024), (1, 1024), (0, 256)])
+def test_churn_batched_patterns_and_counts(gpu, partitions, threads):
     """The workgroup-batched path serves the same stream: every pair
     completes, every pattern reads back, the pool drains to fresh."""
     import torch
     n, rounds, seed = 1 << 20, 10, 10
     a = gpu.Gallatin(S, 16, 4096).generate_on_device(
         8 << 30, 7, print_info=False, slots_min=32, partitions=partitions)
-    ctr = a.run_churn_batched(seed, n, rounds)
+    ctr = a.run_churn_batched(seed, n, rounds, threads=threads)
     torch.cuda.synchronize()
     c = ctr.cpu().tolist()
     assert c[gpu.CTR_BAD] == 0

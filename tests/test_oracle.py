@@ -287,6 +287,30 @@ def test_exhaustion_returns_null_and_recovers():
     assert a.errors == 0
 
 
+def test_class_allocates_again_after_running_out():
+    """F6: a class whose pinned slots went empty when the pool ran out
+    allocates again once memory is freed (the reference's slot stays null
+    for good), and each failing request gives up at once."""
+    a = oracle.Allocator(256 << 20)
+    got = []
+    while True:
+        o = a.malloc(2048)
+        if o is None:
+            break
+        got.append(o)
+    assert len(got) > 4096            # at least one block's worth was served
+    assert a.malloc(2048) is None     # still out of memory
+    for o in got:
+        a.free(o)
+    again = a.malloc(2048)
+    assert again is not None
+    a.free(again)
+    a.drain()
+    assert a.errors == 0
+    ids, act = a.segment_state()
+    assert np.all(ids == 0xFFFF) and np.all(act == -1)
+
+
 def test_invalid_frees_counted():
     a = oracle.Allocator(1 << 30)
     a.free(None)                  # nullptr: ignored
