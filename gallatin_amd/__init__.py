@@ -27,6 +27,7 @@ GAL_FLAG_DETERMINISTIC = 1
 GAL_FLAG_NO_PAYLOAD_FENCE = 2
 
 CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
+CTR_TIMEOUTS, CTR_FREES_SKIPPED = 5, 6
 CTR_COUNT = 8
 
 _u64 = ctypes.c_uint64
@@ -42,7 +43,8 @@ CAPI_SYMBOLS = [
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
     "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
-    "gal_last_error", "gal_get_stats",
+    "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
+    "gal_batch_plan_launch", "gal_batch_plan_destroy",
 ]
 
 
@@ -72,6 +74,8 @@ class Info(ctypes.Structure):
 
 STEP_DTYPE = np.dtype([("kind", "<u4"), ("pad", "<u4"), ("mask", "<u8"),
                        ("arg", "<u8", (64,))])
+# gal_batch_op
+BATCH_OP_DTYPE = np.dtype([("arg", "<u8"), ("kind", "<u4"), ("slot", "<u4")])
 
 _lib = None
 
@@ -110,6 +114,11 @@ def lib():
         L.gal_run_churn_batched.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _p]
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
+        L.gal_run_batches.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32, _u32, _p]
+        L.gal_batch_plan_create.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32,
+                                            _u32, ctypes.POINTER(_p)]
+        L.gal_batch_plan_launch.argtypes = [_p, _p]
+        L.gal_batch_plan_destroy.argtypes = [_p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -291,6 +300,62 @@ class Gallatin:
         _check(lib().gal_run_churn_batched(self.handle, seed, nthreads, rounds,
                                            ctr.data_ptr(), workgroups, threads, _stream()))
         return ctr
+
+
+class BatchStream:
+    """A batched request stream resident on the device: ops (BATCH_OP_DTYPE),
+    batch offsets (host), and the pointer table the ops address."""
+
+    def __init__(self, ops, offsets, nslots):
+        import torch
+        ops = np.ascontiguousarray(ops, dtype=BATCH_OP_DTYPE)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.nbatches = len(self.offsets) - 1
+        self.nslots = int(nslots)
+        self.n_ops = len(ops)
+        self.n_mallocs = int((ops["kind"] == 0).sum())
+        raw = torch.from_numpy(ops.view(np.uint8).copy())
+        self.ops = raw.cuda() if len(ops) else torch.zeros(16, dtype=torch.uint8, device="cuda")
+        self.ptrs = _dev_u64(self.nslots)
+
+    def _offsets_ptr(self):
+        return self.offsets.ctypes.data_as(ctypes.c_void_p)
+
+
+class BatchPlan:
+    """gal_batch_plan: a stream's launches captured in one hipGraph."""
+
+    def __init__(self, alloc, stream, counters, threads=0, batched=True, workgroups=0):
+        self.stream = stream  # keeps the buffers alive
+        self.handle = _p()
+        _check(lib().gal_batch_plan_create(
+            alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(), stream.nbatches,
+            stream.ptrs.data_ptr(), stream.nslots, counters.data_ptr(), threads,
+            int(bool(batched)), workgroups, ctypes.byref(self.handle)))
+
+    def launch(self):
+        _check(lib().gal_batch_plan_launch(self.handle, _stream()))
+
+    def destroy(self):
+        if self.handle:
+            lib().gal_batch_plan_destroy(self.handle)
+            self.handle = _p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def run_batches(alloc, stream, counters=None, threads=0, batched=True, workgroups=0):
+    """Enqueue every batch of `stream` and the cleanup (gal_run_batches)."""
+    ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+    _check(lib().gal_run_batches(alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(),
+                                 stream.nbatches, stream.ptrs.data_ptr(), stream.nslots,
+                                 ctr.data_ptr(), threads, int(bool(batched)), workgroups,
+                                 _stream()))
+    return ctr
 
 
 # ---- global allocator (global_allocator.cuh:41-97) ----

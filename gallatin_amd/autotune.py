@@ -1,0 +1,132 @@
+"""Autotuner for batched request streams.
+
+Sweeps the allocator geometry and the workgroup shape of the batched path
+over one stream and reports pairs/s for each point, beside the plain
+per-thread path at the same geometry:
+
+    segment bytes  {4, 16, 64} MiB
+    smallest slice {16, 128} B
+    workgroup      {256, 512, 1024} threads
+
+The biggest slice is min(segment / 4096, 4096) bytes: a segment must hold
+one block of the biggest class.  Each point replays the stream as one
+captured hipGraph (all batches plus the final cleanup) and checks the
+counters: every malloc issued, every successful malloc freed exactly once,
+every tag intact.
+"""
+import itertools
+
+import numpy as np
+
+from . import (CTR_BAD, CTR_COUNT, CTR_FREES_SKIPPED, CTR_MALLOCS, CTR_MISSES, CTR_PAIRS,
+               CTR_TIMEOUTS, BatchPlan)
+from . import Gallatin, GallatinError
+
+SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
+SMALLEST = (16, 128)
+THREADS = (256, 512, 1024)
+
+
+def biggest_for(segment_bytes):
+    return min(segment_bytes // 4096, 4096)
+
+
+def measure(alloc, stream, batched, threads, reps=3, warmup=1):
+    """Time `reps` graph replays of `stream`; returns a result dict."""
+    import torch
+    ctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
+    plan = BatchPlan(alloc, stream, ctr, threads=threads, batched=batched)
+    try:
+        for _ in range(warmup):
+            plan.launch()
+        torch.cuda.synchronize()
+        ctr.zero_()
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        evs[0].record()
+        for k in range(reps):
+            plan.launch()
+            evs[k + 1].record()
+        torch.cuda.synchronize()
+    finally:
+        plan.destroy()
+    c = ctr.cpu().tolist()
+    total_ms = evs[0].elapsed_time(evs[-1])
+    steps = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(reps))
+    ok = (c[CTR_MALLOCS] == reps * stream.n_mallocs and c[CTR_BAD] == 0
+          and c[CTR_TIMEOUTS] == 0
+          and c[CTR_PAIRS] == c[CTR_MALLOCS] - c[CTR_MISSES]
+          and (c[CTR_MISSES] > 0 or c[CTR_FREES_SKIPPED] == 0))
+    return {
+        "pairs_per_s": c[CTR_PAIRS] / (total_ms / 1e3),
+        "ms_per_step": total_ms / reps,
+        "p50_step_ms": steps[len(steps) // 2],
+        "p99_step_ms": steps[-1],
+        "misses": c[CTR_MISSES],          # over all reps
+        "violations": c[CTR_BAD] + c[CTR_TIMEOUTS] + (0 if ok else 1),
+        "pairs_per_step": c[CTR_PAIRS] / reps,
+    }
+
+
+def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
+         smallest=SMALLEST, threads=THREADS, slots_min=32, flags=0, log=None):
+    """Sweep every (segment, smallest, workgroup) point.  Returns (best,
+    table): best is the fastest batched point with no violations."""
+    table = []
+    for seg, mn in itertools.product(segments, smallest):
+        big = biggest_for(seg)
+        if mn > big:
+            continue
+        alloc = Gallatin(seg, mn, big).generate_on_device(
+            pool_bytes, seed, print_info=False, slots_min=slots_min, flags=flags)
+        try:
+            for wg in threads:
+                r = measure(alloc, stream, True, wg, reps, warmup)
+                r.update(path="batched", segment_bytes=seg, smallest=mn, biggest=big,
+                         threads=wg)
+                table.append(r)
+                if log:
+                    log(r)
+            r = measure(alloc, stream, False, 256, reps, warmup)
+            r.update(path="plain", segment_bytes=seg, smallest=mn, biggest=big, threads=256)
+            table.append(r)
+            if log:
+                log(r)
+            if alloc.info()["errors"]:
+                raise GallatinError(f"allocator errors at segment {seg}, smallest {mn}")
+        finally:
+            alloc.free_on_device()
+    good = [r for r in table if r["path"] == "batched" and r["violations"] == 0]
+    best = max(good, key=lambda r: r["pairs_per_s"]) if good else None
+    return best, table
+
+
+def main(argv=None):
+    """python -m gallatin_amd.autotune [--seed S] [--pool-gib G] [--reps R]:
+    generate the Zipf stream, sweep, print one JSON line per point and the
+    best point last."""
+    import argparse
+    import json
+
+    import torch
+
+    from . import BatchStream
+    from .batch_streams import zipf_stream
+
+    p = argparse.ArgumentParser()
+    p.add_argument("--seed", type=int, default=1234)
+    p.add_argument("--pool-gib", type=int, default=64)
+    p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--nbatches", type=int, default=256)
+    p.add_argument("--flags", type=int, default=0)
+    a = p.parse_args(argv)
+    torch.cuda.set_device(0)
+    ops, off, nslots = zipf_stream(a.seed, nbatches=a.nbatches)
+    stream = BatchStream(ops, off, nslots)
+    best, _ = tune(stream, a.pool_gib << 30, seed=a.seed, reps=a.reps, flags=a.flags,
+                   log=lambda r: print(json.dumps(r), flush=True))
+    print(json.dumps({"best": best}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

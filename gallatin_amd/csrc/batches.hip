@@ -1,0 +1,505 @@
+// Batched request streams: batches of mixed malloc and free requests, one
+// persistent launch per batch, with the launches of a whole stream captured
+// in one hipGraph and replayed.
+//
+// An op is a malloc of `arg` bytes whose pointer goes to ptrs[slot], or a
+// free of the pointer in ptrs[slot] (written by an earlier batch; nullptr,
+// from a malloc that failed, is skipped).  Every allocation holds its slot
+// number in its first 8 bytes, checked when it is freed.  After the last
+// batch a cleanup launch frees every pointer still in the table, so a
+// replay starts from an empty table.
+//
+// Two ways to serve a batch: the workgroup-batched path (batch_step of
+// batched.hpp) and the plain per-thread malloc / free of gallatin.hpp.  No
+// reference counterpart; the reference's tests drive the allocator with one
+// kernel per operation type (tests/src/gallatin_test.cu:59-112).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "gallatin/allocators/batched.hpp"
+#include "gallatin/allocators/gallatin.hpp"
+#include "gallatin_capi.h"
+#include "internal.hpp"
+
+using namespace gallatin;
+using namespace gallatin::allocators;
+using namespace gallatin_capi;
+
+namespace {
+
+__device__ inline uint64_t wave_total(uint64_t v) {
+  for (int off = 32; off >= 1; off >>= 1)
+    v += wave::fetch(v, (int)(wave::lane() ^ (uint32_t)off));
+  return v;
+}
+
+struct op_counts {
+  uint64_t pairs = 0, misses = 0, bad = 0, bytes = 0, mallocs = 0, skipped = 0;
+};
+
+// One atomic per counter per wave (every lane of the wave active).
+__device__ inline void flush_counts(uint64_t* ctr, const op_counts& c) {
+  const uint64_t v[5] = {wave_total(c.pairs), wave_total(c.misses), wave_total(c.bad),
+                         wave_total(c.bytes), wave_total(c.mallocs)};
+  const uint64_t sk = wave_total(c.skipped);
+  if (wave::lane() == 0) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      if (v[i]) utils::atomic_add(&ctr[i], v[i]);
+    if (sk) utils::atomic_add(&ctr[GAL_CTR_FREES_SKIPPED], sk);
+  }
+}
+
+// Pointer table protocol.  An entry is 0 until its malloc has run, then
+// the pointer, or kMallocFailed when the malloc returned nullptr.  A free
+// waits (bounded) for its entry to leave 0, so it never depends on launch
+// boundaries: batches may run in one launch.  Entries and tags cross
+// workgroups and XCDs through agent-scope atomic loads and stores (coherent
+// at the device, not held in a non-coherent cache), the tag store is
+// acknowledged before the entry is published, and the free reads the tag
+// only after it has seen the entry.
+constexpr uint64_t kMallocFailed = 1;
+
+// Decode op i: the pointer to free (taken out of the table, tag checked) or
+// the size to allocate.
+__device__ inline void take_op(const gal_batch_op* ops, uint64_t i, bool active,
+                               uint64_t* ptrs, uint64_t nslots, op_counts& c, void*& old,
+                               bool& want, uint64_t& size, uint32_t& slot,
+                               uint32_t spin_cap, uint64_t* ctr) {
+  old = nullptr;
+  want = false;
+  size = 0;
+  slot = 0;
+  if (!active) return;
+  const gal_batch_op op = ops[i];
+  if (op.slot >= nslots || op.kind > 1) {  // not a valid op: counted, skipped
+    c.bad += 1;
+    return;
+  }
+  slot = op.slot;
+  if (op.kind == 0) {
+    want = true;
+    size = op.arg;
+    return;
+  }
+  uint64_t v = utils::atomic_load(&ptrs[slot]);
+  for (uint32_t spins = 0; v == 0; ++spins) {  // its malloc has not run yet
+    if (spins >= spin_cap) {
+      utils::atomic_add(&ctr[GAL_CTR_TIMEOUTS], (uint64_t)1);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(4);
+    v = utils::atomic_load(&ptrs[slot]);
+  }
+  utils::atomic_store(&ptrs[slot], (uint64_t)0);
+  if (v == kMallocFailed) {
+    c.skipped += 1;
+    return;
+  }
+  uint64_t* p = reinterpret_cast<uint64_t*>(v);
+  c.bad += utils::atomic_load(p) != (uint64_t)slot;
+  c.bytes += 16;  // tag written at malloc, read here
+  c.pairs += 1;
+  old = p;
+}
+
+__device__ inline void give_op(void* np, bool want, uint32_t slot, uint64_t* ptrs,
+                               op_counts& c) {
+  if (want) {
+    c.mallocs += 1;
+    if (np == nullptr) c.misses += 1;
+    if (np) utils::atomic_store(static_cast<uint64_t*>(np), (uint64_t)slot);
+  }
+  // the tag is acknowledged before the entry is published
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  if (want)
+    utils::atomic_store(&ptrs[slot], np ? reinterpret_cast<uint64_t>(np) : kMallocFailed);
+}
+
+// One batch through the workgroup-batched path: a persistent grid walks
+// chunks of kThreads ops.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_batch_batched(gal_state* s, const gal_batch_op* ops,
+                                                            uint64_t n, uint64_t* ptrs,
+                                                            uint64_t nslots, uint64_t* ctr,
+                                                            uint32_t spin_cap) {
+  __shared__ batch::lds_state lds;
+  batch::batch_init(lds);
+  op_counts c;
+  const uint64_t nchunks = (n + kThreads - 1) / kThreads;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i = chunk * kThreads + threadIdx.x;
+    void* old;
+    bool want;
+    uint64_t size;
+    uint32_t slot;
+    take_op(ops, i, i < n, ptrs, nslots, c, old, want, size, slot, spin_cap, ctr);
+    void* np = batch::batch_step(s, old, want, size, lds);
+    give_op(np, want, slot, ptrs, c);
+  }
+  flush_counts(ctr, c);
+}
+
+// The same batch through the per-thread path.
+__global__ __launch_bounds__(256) void k_batch_plain(gal_state* s, const gal_batch_op* ops,
+                                                     uint64_t n, uint64_t* ptrs, uint64_t nslots,
+                                                     uint64_t* ctr, uint32_t spin_cap) {
+  op_counts c;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t rounds = (n + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    void* old;
+    bool want;
+    uint64_t size;
+    uint32_t slot;
+    take_op(ops, i, i < n, ptrs, nslots, c, old, want, size, slot, spin_cap, ctr);
+    if (old) detail::free_impl(s, old);
+    void* np = nullptr;
+    if (want) np = detail::malloc_impl(s, size);
+    give_op(np, want, slot, ptrs, c);
+  }
+  flush_counts(ctr, c);
+}
+
+// Cleanup: free every pointer left in the table and clear every entry.
+__device__ inline void take_left(uint64_t* ptrs, uint64_t i, bool in, op_counts& c,
+                                 void*& old) {
+  old = nullptr;
+  if (!in) return;
+  const uint64_t v = ptrs[i];
+  if (v == 0) return;
+  ptrs[i] = 0;
+  if (v == kMallocFailed) return;
+  uint64_t* p = reinterpret_cast<uint64_t*>(v);
+  c.bad += *p != i;
+  c.bytes += 16;
+  c.pairs += 1;
+  old = p;
+}
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_table_free_batched(gal_state* s, uint64_t* ptrs,
+                                                                 uint64_t nslots, uint64_t* ctr) {
+  __shared__ batch::lds_state lds;
+  batch::batch_init(lds);
+  op_counts c;
+  const uint64_t nchunks = (nslots + kThreads - 1) / kThreads;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i = chunk * kThreads + threadIdx.x;
+    void* old;
+    take_left(ptrs, i, i < nslots, c, old);
+    batch::batch_free(s, old, lds);
+  }
+  flush_counts(ctr, c);
+}
+
+__global__ __launch_bounds__(256) void k_table_free_plain(gal_state* s, uint64_t* ptrs,
+                                                          uint64_t nslots, uint64_t* ctr) {
+  op_counts c;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t rounds = (nslots + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t i = r * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    void* old;
+    take_left(ptrs, i, i < nslots, c, old);
+    if (old) detail::free_impl(s, old);
+  }
+  flush_counts(ctr, c);
+}
+
+// ---- one persistent launch for a whole stream ----
+//
+// The stream is cut into chunks of at most kChunkRounds * kThreads ops that
+// never span two batches, so a free never waits on a malloc of its own
+// chunk.  Workgroups take chunks in stream order from a queue counter and
+// run them at once: a free waits only for its own table entry.  A chunk is
+// only taken by a running workgroup and a free's malloc lies in a chunk
+// taken earlier, so the earliest unfinished chunk never waits and the grid
+// always drains; the waits are bounded all the same (counter [5]).
+struct stream_chunk {
+  uint64_t begin;  // first op
+  uint32_t count;  // ops
+  uint32_t batch;
+};
+
+constexpr uint32_t kChunkRounds = 4;
+
+template <int kThreads, bool kBatched>
+__global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_batch_op* ops,
+                                                     const stream_chunk* chunks, uint32_t nchunks,
+                                                     uint32_t* queue, uint64_t* ptrs,
+                                                     uint64_t nslots, uint64_t* ctr,
+                                                     uint32_t spin_cap) {
+  __shared__ batch::lds_state lds;
+  __shared__ uint32_t cur;
+  if (kBatched) batch::batch_init(lds);
+  op_counts c;
+  for (;;) {
+    if (threadIdx.x == 0) cur = utils::atomic_add(queue, 1u);
+    __syncthreads();
+    const uint32_t k = cur;
+    __syncthreads();  // every thread has read cur before it is written again
+    if (k >= nchunks) break;
+    const stream_chunk ch = chunks[k];
+    const uint32_t rounds = (ch.count + kThreads - 1) / kThreads;
+    for (uint32_t r = 0; r < rounds; ++r) {
+      const uint32_t j = r * kThreads + threadIdx.x;
+      void* old;
+      bool want;
+      uint64_t size;
+      uint32_t slot;
+      take_op(ops, ch.begin + j, j < ch.count, ptrs, nslots, c, old, want, size, slot,
+              spin_cap, ctr);
+      void* np = nullptr;
+      if (kBatched) {
+        np = batch::batch_step(s, old, want, size, lds);
+      } else {
+        if (old) detail::free_impl(s, old);
+        if (want) np = detail::malloc_impl(s, size);
+      }
+      give_op(np, want, slot, ptrs, c);
+    }
+  }
+  flush_counts(ctr, c);
+}
+
+__global__ void k_stream_reset(uint32_t* queue) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = 0;
+}
+
+template <typename K>
+unsigned fill_grid(K kernel, int threads, uint32_t workgroups) {
+  if (workgroups) return workgroups;
+  int dev = 0, cus = 256, per_cu = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (unsigned)(cus * per_cu);
+}
+
+constexpr uint32_t kStreamSpinCap = 1u << 22;  // polls of a free waiting for its entry
+
+template <int kThreads>
+void launch_batched(gal_state* s, const gal_batch_op* ops, uint64_t n, uint64_t* ptrs,
+                    uint64_t nslots, uint64_t* ctr, uint32_t workgroups, hipStream_t st) {
+  unsigned g = fill_grid(k_batch_batched<kThreads>, kThreads, workgroups);
+  const uint64_t chunks = (n + kThreads - 1) / kThreads;
+  if (g > chunks) g = (unsigned)chunks;
+  k_batch_batched<kThreads><<<g, kThreads, 0, st>>>(s, ops, n, ptrs, nslots, ctr, kStreamSpinCap);
+}
+
+template <int kThreads>
+void launch_cleanup(gal_state* s, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                    uint32_t workgroups, hipStream_t st) {
+  unsigned g = fill_grid(k_table_free_batched<kThreads>, kThreads, workgroups);
+  const uint64_t chunks = (nslots + kThreads - 1) / kThreads;
+  if (g > chunks) g = (unsigned)chunks;
+  k_table_free_batched<kThreads><<<g, kThreads, 0, st>>>(s, ptrs, nslots, ctr);
+}
+
+}  // namespace
+
+struct gal_batch_plan {
+  gal_allocator* a = nullptr;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // persistent-launch state (device)
+  stream_chunk* chunks = nullptr;
+  uint32_t* queue = nullptr;
+};
+
+// Enqueue reset, the persistent launch and the cleanup on `st`.
+static int enqueue_persistent(gal_batch_plan* plan, const gal_batch_op* ops, uint32_t nchunks,
+                              uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                              uint32_t threads, uint32_t batched, uint32_t workgroups,
+                              hipStream_t st) {
+  gal_state* s = plan->a->dev;
+  k_stream_reset<<<1, 64, 0, st>>>(plan->queue);
+  if (nchunks) {
+#define GAL_STREAM_LAUNCH(T, B)                                                              \
+  do {                                                                                        \
+    unsigned g = fill_grid(k_stream<T, B>, T, workgroups);                                    \
+    if (g > nchunks) g = nchunks;                                                             \
+    k_stream<T, B><<<g, T, 0, st>>>(s, ops, plan->chunks, nchunks, plan->queue, ptrs, nslots, \
+                                    ctr, kStreamSpinCap);                                          \
+  } while (0)
+    if (!batched)
+      GAL_STREAM_LAUNCH(256, false);
+    else if (threads == 256)
+      GAL_STREAM_LAUNCH(256, true);
+    else if (threads == 512)
+      GAL_STREAM_LAUNCH(512, true);
+    else
+      GAL_STREAM_LAUNCH(1024, true);
+#undef GAL_STREAM_LAUNCH
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "stream launch");
+  if (nslots) {
+    if (!batched) {
+      unsigned g = fill_grid(k_table_free_plain, 256, workgroups);
+      const uint64_t need = (nslots + 255) / 256;
+      if (g > need) g = (unsigned)need;
+      k_table_free_plain<<<g, 256, 0, st>>>(s, ptrs, nslots, ctr);
+    } else if (threads == 256) {
+      launch_cleanup<256>(s, ptrs, nslots, ctr, workgroups, st);
+    } else if (threads == 512) {
+      launch_cleanup<512>(s, ptrs, nslots, ctr, workgroups, st);
+    } else {
+      launch_cleanup<1024>(s, ptrs, nslots, ctr, workgroups, st);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "cleanup launch");
+  }
+  return GAL_OK;
+}
+
+// Enqueue every batch and the cleanup on `st`.
+static int enqueue_stream(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
+                          uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                          uint32_t threads, uint32_t batched, uint32_t workgroups,
+                          hipStream_t st) {
+  gal_state* s = a->dev;
+  for (uint32_t b = 0; b < nbatches; ++b) {
+    const uint64_t n = offsets[b + 1] - offsets[b];
+    if (n == 0) continue;
+    const gal_batch_op* o = ops + offsets[b];
+    if (!batched) {
+      unsigned g = fill_grid(k_batch_plain, 256, workgroups);
+      const uint64_t need = (n + 255) / 256;
+      if (g > need) g = (unsigned)need;
+      k_batch_plain<<<g, 256, 0, st>>>(s, o, n, ptrs, nslots, ctr, kStreamSpinCap);
+    } else if (threads == 256) {
+      launch_batched<256>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    } else if (threads == 512) {
+      launch_batched<512>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    } else {
+      launch_batched<1024>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+  }
+  if (nslots) {
+    if (!batched) {
+      unsigned g = fill_grid(k_table_free_plain, 256, workgroups);
+      const uint64_t need = (nslots + 255) / 256;
+      if (g > need) g = (unsigned)need;
+      k_table_free_plain<<<g, 256, 0, st>>>(s, ptrs, nslots, ctr);
+    } else if (threads == 256) {
+      launch_cleanup<256>(s, ptrs, nslots, ctr, workgroups, st);
+    } else if (threads == 512) {
+      launch_cleanup<512>(s, ptrs, nslots, ctr, workgroups, st);
+    } else {
+      launch_cleanup<1024>(s, ptrs, nslots, ctr, workgroups, st);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "cleanup launch");
+  }
+  return GAL_OK;
+}
+
+static int check_plan_args(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
+                           uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                           uint32_t* threads) {
+  if (!a || !offsets || !ctr || (nslots && !ptrs)) return fail(GAL_ERR_INVALID, "null argument");
+  if (nbatches && offsets[nbatches] && !ops) return fail(GAL_ERR_INVALID, "null ops");
+  for (uint32_t b = 0; b < nbatches; ++b)
+    if (offsets[b + 1] < offsets[b]) return fail(GAL_ERR_INVALID, "offsets must not decrease");
+  if (nslots > 0xFFFFFFFFull) return fail(GAL_ERR_INVALID, "at most 2^32 slots");
+  if (*threads == 0) *threads = 1024;
+  if (*threads != 256 && *threads != 512 && *threads != 1024)
+    return fail(GAL_ERR_INVALID, "batched streams run 256, 512 or 1024 threads per workgroup");
+  return GAL_OK;
+}
+
+extern "C" {
+
+int gal_run_batches(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
+                    uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                    uint32_t threads, uint32_t batched, uint32_t workgroups, void* stream) {
+  const int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads);
+  if (rc != GAL_OK) return rc;
+  return enqueue_stream(a, ops, offsets, nbatches, ptrs, nslots, ctr, threads, batched,
+                        workgroups, (hipStream_t)stream);
+}
+
+int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
+                          uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                          uint32_t threads, uint32_t batched, uint32_t workgroups,
+                          gal_batch_plan** out) {
+  if (!out) return fail(GAL_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads);
+  if (rc != GAL_OK) return rc;
+  // chunk table: at most kChunkRounds * threads ops, never across batches
+  const uint32_t per = kChunkRounds * (batched ? threads : 256u);
+  std::vector<stream_chunk> chunks;
+  for (uint32_t b = 0; b < nbatches; ++b) {
+    for (uint64_t at = offsets[b]; at < offsets[b + 1]; at += per) {
+      const uint64_t left = offsets[b + 1] - at;
+      chunks.push_back({at, (uint32_t)(left < per ? left : per), b});
+    }
+  }
+  if (chunks.size() > 0xFFFFFFF0ull) return fail(GAL_ERR_INVALID, "stream too long");
+  gal_batch_plan* plan = new gal_batch_plan();
+  plan->a = a;
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&plan->chunks, (chunks.size() + 1) * sizeof(stream_chunk))) != hipSuccess ||
+      (e = hipMalloc(&plan->queue, 4 * sizeof(uint32_t))) != hipSuccess ||
+      (!chunks.empty() &&
+       (e = hipMemcpy(plan->chunks, chunks.data(), chunks.size() * sizeof(stream_chunk),
+                      hipMemcpyHostToDevice)) != hipSuccess)) {
+    gal_batch_plan_destroy(plan);
+    return hip_fail(e, "plan buffers");
+  }
+  e = hipStreamCreateWithFlags(&plan->capture, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    gal_batch_plan_destroy(plan);
+    return hip_fail(e, "capture stream");
+  }
+  e = hipStreamBeginCapture(plan->capture, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    gal_batch_plan_destroy(plan);
+    return hip_fail(e, "begin capture");
+  }
+  rc = enqueue_persistent(plan, ops, (uint32_t)chunks.size(), nbatches, ptrs, nslots, ctr,
+                          threads, batched, workgroups, plan->capture);
+  e = hipStreamEndCapture(plan->capture, &plan->graph);
+  if (rc == GAL_OK && e != hipSuccess) rc = hip_fail(e, "end capture");
+  if (rc == GAL_OK) {
+    e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "graph instantiate");
+  }
+  if (rc != GAL_OK) {
+    gal_batch_plan_destroy(plan);
+    return rc;
+  }
+  *out = plan;
+  return GAL_OK;
+}
+
+int gal_batch_plan_launch(gal_batch_plan* plan, void* stream) {
+  if (!plan || !plan->exec) return fail(GAL_ERR_INVALID, "null plan");
+  const hipError_t e = hipGraphLaunch(plan->exec, (hipStream_t)stream);
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "graph launch");
+}
+
+int gal_batch_plan_destroy(gal_batch_plan* plan) {
+  if (!plan) return GAL_OK;
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->capture) (void)hipStreamDestroy(plan->capture);
+  if (plan->chunks) (void)hipFree(plan->chunks);
+  if (plan->queue) (void)hipFree(plan->queue);
+  delete plan;
+  return GAL_OK;
+}
+
+}  // extern "C"

@@ -178,9 +178,12 @@ int gal_run_wave_stream(gal_allocator *a, const gal_wave_step *steps_dev,
 
 /* Counters written by the workload kernels (device array, 8 x u64):
  * [0] pairs (successful malloc+free), [1] misses (nullptr), [2] pattern
- * mismatches, [3] payload bytes written+read, [4] mallocs issued. */
+ * mismatches, [3] payload bytes written+read, [4] mallocs issued, [5]
+ * bounded waits that ran out (batched streams), [6] frees of a table entry
+ * that held nullptr (batched streams: the malloc had failed). */
 enum { GAL_CTR_PAIRS = 0, GAL_CTR_MISSES = 1, GAL_CTR_BAD = 2,
-       GAL_CTR_BYTES = 3, GAL_CTR_MALLOCS = 4, GAL_CTR_COUNT = 8 };
+       GAL_CTR_BYTES = 3, GAL_CTR_MALLOCS = 4, GAL_CTR_TIMEOUTS = 5,
+       GAL_CTR_FREES_SKIPPED = 6, GAL_CTR_COUNT = 8 };
 
 /* Plain method: nthreads threads each malloc(size), write their uint32 id,
  * read it back and free (config "plain small"; gallatin_test.cu:59-112 in
@@ -205,6 +208,46 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
 int gal_run_churn_batched(gal_allocator *a, uint64_t seed, uint64_t nthreads,
                           uint32_t rounds, uint64_t *counters_dev,
                           uint32_t workgroups, uint32_t threads, void *stream);
+
+/* ---- batched request streams (no reference counterpart) ----
+ * A stream is nbatches batches of ops; batch b is ops[offsets[b] ..
+ * offsets[b+1]) (offsets is a host array of nbatches + 1 entries).  kind 0:
+ * malloc(arg) into ptrs[slot]; kind 1: free(ptrs[slot]) (nullptr skipped),
+ * where slot was filled by an earlier batch.  Each allocation holds its slot
+ * number in its first 8 bytes, checked at free (counter [2]; an op with a
+ * slot >= nslots or an unknown kind also counts there and is skipped).  One
+ * persistent launch per batch (the workgroup-batched path when `batched`,
+ * else plain per-thread malloc/free), then one launch freeing whatever is
+ * left in ptrs[0 .. nslots), so the table is empty again at the end.
+ * threads: 256, 512 or 1024 (0 = 1024, batched path only); workgroups: 0 =
+ * fill the device.  Counters as for the workload kernels. */
+typedef struct gal_batch_op {
+  uint64_t arg;   /* malloc: bytes */
+  uint32_t kind;  /* 0 malloc, 1 free */
+  uint32_t slot;  /* pointer table entry */
+} gal_batch_op;
+
+int gal_run_batches(gal_allocator *a, const gal_batch_op *ops_dev,
+                    const uint64_t *offsets, uint32_t nbatches, uint64_t *ptrs_dev,
+                    uint64_t nslots, uint64_t *counters_dev, uint32_t threads,
+                    uint32_t batched, uint32_t workgroups, void *stream);
+
+/* The stream as one persistent launch, captured once into a hipGraph (on a
+ * stream of its own: a reset kernel, the persistent kernel, the cleanup)
+ * and replayed with gal_batch_plan_launch.  Workgroups take chunks of at
+ * most 4 x threads ops (never across batches) in stream order from a
+ * device queue; a chunk of batch b waits until every chunk of batch b - 1
+ * is done (bounded: a wait that runs out counts in counter [5]).  The
+ * buffers must stay valid until the plan is destroyed. */
+typedef struct gal_batch_plan gal_batch_plan;
+int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
+                          const uint64_t *offsets, uint32_t nbatches,
+                          uint64_t *ptrs_dev, uint64_t nslots,
+                          uint64_t *counters_dev, uint32_t threads,
+                          uint32_t batched, uint32_t workgroups,
+                          gal_batch_plan **out);
+int gal_batch_plan_launch(gal_batch_plan *plan, void *stream);
+int gal_batch_plan_destroy(gal_batch_plan *plan);
 
 /* ---- measurement kernels (ceilings for the roofline) ---- */
 /* n_ops relaxed agent-scope atomicAdd(u32) with the returned value kept:

@@ -237,3 +237,19 @@ def churn_size(seed, tids, r):
 
 def log_uniform_mean_bytes():
     return 4095 / math.log(4096)
+
+
+def check_fresh(a):
+    """Metadata of a drained instance (nothing allocated, nothing pinned)
+    equals the freshly initialised state bit for bit."""
+    st, sl, bl = a.snapshot()
+    assert np.all(st == np.uint64(0xFFFFFFFF0000FFFF))
+    assert np.all(sl == 0xFFFFFFFF)
+    assert np.all(bl[:, 1] == 0), "a block kept freed slices"
+    for x, y in zip(a.tree_layers(-1), fresh_layers(a.num_segments, True)):
+        assert np.array_equal(x, y)
+    for t in range(a.num_trees):
+        for p in range(a.partitions):
+            for x in a.tree_layers(t, p):
+                assert not np.any(x)
+    assert a.info()["errors"] == 0

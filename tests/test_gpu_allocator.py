@@ -40,19 +40,7 @@ def compare_metadata(a, o):
                    zip(a.tree_layers(t), o.sub_tree(t).layers())), f"class tree {t} differs"
 
 
-def check_fresh(a):
-    """Metadata of an instance with nothing allocated and nothing pinned."""
-    st, sl, bl = a.snapshot()
-    assert np.all(st == np.uint64(0xFFFFFFFF0000FFFF))
-    assert np.all(sl == 0xFFFFFFFF)
-    assert np.all(bl[:, 1] == 0), "a block kept freed slices"
-    for x, y in zip(a.tree_layers(-1), H.fresh_layers(a.num_segments, True)):
-        assert np.array_equal(x, y)
-    for t in range(a.num_trees):
-        for p in range(a.partitions):
-            for x in a.tree_layers(t, p):
-                assert not np.any(x)
-    assert a.info()["errors"] == 0
+check_fresh = H.check_fresh
 
 
 def test_boot_matches_oracle_and_golden(gpu):

@@ -1,0 +1,72 @@
+"""Batched request streams on the GPU: the workgroup-batched and the plain
+path serve the same Zipf stream, eagerly and as replays of one captured
+hipGraph; every replay issues every malloc, frees every successful one
+exactly once with its tag intact, leaves the pointer table empty, and the
+drained instance equals the fresh state."""
+import numpy as np
+import pytest
+
+from gallatin_amd import batch_streams as B
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_stream(gpu):
+    ops, off, nslots = B.zipf_stream(11, nbatches=48, max_count=1 << 13)
+    B.check_stream(ops, off, nslots)
+    return gpu.BatchStream(ops, off, nslots)
+
+
+def _check(gpu, c, stream, reps=1, misses_ok=False):
+    assert c[gpu.CTR_MALLOCS] == reps * stream.n_mallocs
+    assert c[gpu.CTR_BAD] == 0
+    assert c[gpu.CTR_TIMEOUTS] == 0
+    assert c[gpu.CTR_PAIRS] == c[gpu.CTR_MALLOCS] - c[gpu.CTR_MISSES]
+    if not misses_ok:
+        assert c[gpu.CTR_MISSES] == 0
+        # every free found its pointer: the table writes of earlier batches
+        # were visible to it
+        assert c[gpu.CTR_FREES_SKIPPED] == 0
+
+
+@pytest.mark.parametrize("seg,mn,batched,threads", [
+    (16 << 20, 16, True, 1024), (16 << 20, 16, True, 256), (16 << 20, 16, False, 256),
+    (4 << 20, 128, True, 512), (64 << 20, 16, True, 1024)])
+def test_eager_stream(gpu, small_stream, seg, mn, batched, threads):
+    import torch
+    a = gpu.Gallatin(seg, mn, min(seg // 4096, 4096)).generate_on_device(
+        8 << 30, 5, print_info=False)
+    ctr = gpu.run_batches(a, small_stream, threads=threads, batched=batched)
+    torch.cuda.synchronize()
+    _check(gpu, ctr.cpu().tolist(), small_stream)
+    assert not torch.any(small_stream.ptrs != 0), "pointer table not empty"
+    a.drain()
+    H.check_fresh(a)
+    a.free_on_device()
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_graph_replays(gpu, small_stream, batched):
+    """Three replays of the captured graph, each followed by eager work on
+    the same instance (an eager run of the stream) and a synchronisation;
+    every replay is checked on its own."""
+    import torch
+    a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(8 << 30, 6, print_info=False)
+    ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
+    plan = gpu.BatchPlan(a, small_stream, ctr, batched=batched)
+    for rep in range(3):
+        ctr.zero_()
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        _check(gpu, ctr.cpu().tolist(), small_stream)
+        assert not torch.any(small_stream.ptrs != 0), f"replay {rep}: table not empty"
+        eager = gpu.run_batches(a, small_stream, batched=not batched)
+        torch.cuda.synchronize()
+        _check(gpu, eager.cpu().tolist(), small_stream)
+    plan.destroy()
+    a.drain()
+    H.check_fresh(a)
+    a.free_on_device()
