@@ -45,6 +45,9 @@ CAPI_SYMBOLS = [
     "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
     "gal_batch_plan_launch", "gal_batch_plan_destroy",
+    "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
+    "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
+    "gal_table_remove", "gal_table_stats",
 ]
 
 
@@ -119,6 +122,17 @@ def lib():
                                             _u32, ctypes.POINTER(_p)]
         L.gal_batch_plan_launch.argtypes = [_p, _p]
         L.gal_batch_plan_destroy.argtypes = [_p]
+        L.gal_vector_create.argtypes = [ctypes.POINTER(_p)]
+        L.gal_vector_destroy.argtypes = [_p]
+        L.gal_vector_insert.argtypes = [_p, _p, _u64, _p, ctypes.c_int, _p]
+        L.gal_vector_export.restype = ctypes.c_int64
+        L.gal_vector_export.argtypes = [_p, _p, _u64]
+        L.gal_table_create.argtypes = [ctypes.c_int, _u64, ctypes.POINTER(_p)]
+        L.gal_table_destroy.argtypes = [_p]
+        L.gal_table_insert.argtypes = [_p, _p, _p, _u64, _p, _p]
+        L.gal_table_query.argtypes = [_p, _p, _u64, _p, _p, _p]
+        L.gal_table_remove.argtypes = [_p, _p, _u64, _p, _p]
+        L.gal_table_stats.argtypes = [_p, _p, _p, _p, _p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -356,6 +370,74 @@ def run_batches(alloc, stream, counters=None, threads=0, batched=True, workgroup
                                  ctr.data_ptr(), threads, int(bool(batched)), workgroups,
                                  _stream()))
     return ctr
+
+
+# ---- allocator-backed data structures (on the global allocator) ----
+class Vector:
+    """fixed_vector<uint64_t, 64, 2^22> (fixed_vector.cuh:72-512)."""
+
+    def __init__(self):
+        self.handle = _p()
+        _check(lib().gal_vector_create(ctypes.byref(self.handle)))
+
+    def insert(self, items, bulk=False):
+        import torch
+        out = torch.empty(items.numel(), dtype=torch.int64, device="cuda")
+        _check(lib().gal_vector_insert(self.handle, items.data_ptr(), items.numel(),
+                                       out.data_ptr(), int(bool(bulk)), _stream()))
+        return out
+
+    def export(self):
+        n = _check(lib().gal_vector_export(self.handle, None, 0))
+        buf = np.zeros(max(n, 1), np.uint64)
+        _check(lib().gal_vector_export(self.handle, buf.ctypes.data_as(ctypes.c_void_p), n))
+        return buf[:n]
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().gal_vector_destroy(self.handle))
+            self.handle = _p()
+
+
+class Table:
+    """extendible_hash_table of uint64 keys / values (extendible_ht.cuh:412);
+    kind 0: 2^4..2^20 buckets, kind 1: 2^16..2^28."""
+
+    def __init__(self, kind=0, seed=42):
+        self.handle = _p()
+        _check(lib().gal_table_create(kind, seed, ctypes.byref(self.handle)))
+
+    def insert(self, keys, vals):
+        import torch
+        ok = torch.zeros(keys.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_table_insert(self.handle, keys.data_ptr(), vals.data_ptr(),
+                                      keys.numel(), ok.data_ptr(), _stream()))
+        return ok
+
+    def query(self, keys):
+        import torch
+        vals = torch.zeros(keys.numel(), dtype=torch.int64, device="cuda")
+        found = torch.zeros(keys.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_table_query(self.handle, keys.data_ptr(), keys.numel(),
+                                     vals.data_ptr(), found.data_ptr(), _stream()))
+        return vals, found
+
+    def remove(self, keys):
+        import torch
+        ok = torch.zeros(keys.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_table_remove(self.handle, keys.data_ptr(), keys.numel(),
+                                      ok.data_ptr(), _stream()))
+        return ok
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(lib().gal_table_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("keys", "buckets", "level", "failed"), (x.value for x in v)))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().gal_table_destroy(self.handle))
+            self.handle = _p()
 
 
 # ---- global allocator (global_allocator.cuh:41-97) ----

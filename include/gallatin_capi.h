@@ -209,6 +209,38 @@ int gal_run_churn_batched(gal_allocator *a, uint64_t seed, uint64_t nthreads,
                           uint32_t rounds, uint64_t *counters_dev,
                           uint32_t workgroups, uint32_t threads, void *stream);
 
+/* ---- allocator-backed data structures (on the global allocator) ----
+ * gal_vector: fixed_vector<uint64_t, 64, 2^22> (data_structs/fixed_vector.cuh
+ * :72-512, header include/gallatin/data_structs/fixed_vector.hpp).  insert
+ * writes items[i] at a fresh position (out_positions[i], ~0 when full);
+ * bulk != 0 takes the positions of a wave with one atomic (bulk_insert).
+ * export copies items [0, size) to the host and returns size. */
+typedef struct gal_vector gal_vector;
+int gal_vector_create(gal_vector **out);
+int gal_vector_destroy(gal_vector *v);
+int gal_vector_insert(gal_vector *v, const uint64_t *items_dev, uint64_t n,
+                      uint64_t *out_positions_dev, int bulk, void *stream);
+int64_t gal_vector_export(gal_vector *v, uint64_t *out_host, uint64_t cap);
+
+/* gal_table: extendible_hash_table<uint64_t, 0, ~0, uint64_t, 15, min, max>
+ * (data_structs/extendible_ht.cuh:412-1437, header
+ * include/gallatin/data_structs/extendible_ht.hpp); kind 0: 2^4 .. 2^20
+ * buckets, kind 1: 2^16 .. 2^28.  Keys 0 and ~0 are reserved.  insert
+ * replaces the value of a present key; remove has no reference
+ * counterpart.  ok / found: one byte per key (may be NULL). */
+typedef struct gal_table gal_table;
+int gal_table_create(int kind, uint64_t seed, gal_table **out);
+int gal_table_destroy(gal_table *t);
+int gal_table_insert(gal_table *t, const uint64_t *keys_dev, const uint64_t *vals_dev,
+                     uint64_t n, uint8_t *ok_dev, void *stream);
+int gal_table_query(gal_table *t, const uint64_t *keys_dev, uint64_t n,
+                    uint64_t *vals_dev, uint8_t *found_dev, void *stream);
+int gal_table_remove(gal_table *t, const uint64_t *keys_dev, uint64_t n,
+                     uint8_t *ok_dev, void *stream);
+/* Keys present, buckets, deepest bucket depth, inserts refused.  Sync. */
+int gal_table_stats(gal_table *t, uint64_t *keys, uint64_t *buckets,
+                    uint64_t *level, uint64_t *failed);
+
 /* ---- batched request streams (no reference counterpart) ----
  * A stream is nbatches batches of ops; batch b is ops[offsets[b] ..
  * offsets[b+1]) (offsets is a host array of nbatches + 1 entries).  kind 0:
