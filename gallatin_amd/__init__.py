@@ -47,7 +47,8 @@ CAPI_SYMBOLS = [
     "gal_batch_plan_launch", "gal_batch_plan_destroy",
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
     "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
-    "gal_table_remove", "gal_table_stats",
+    "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
+    "gal_serving_check", "gal_serving_table", "gal_serving_destroy",
 ]
 
 
@@ -133,6 +134,11 @@ def lib():
         L.gal_table_query.argtypes = [_p, _p, _u64, _p, _p, _p]
         L.gal_table_remove.argtypes = [_p, _p, _u64, _p, _p]
         L.gal_table_stats.argtypes = [_p, _p, _p, _p, _p]
+        L.gal_serving_create.argtypes = [_u32, _u64, _p, ctypes.POINTER(_p)]
+        L.gal_serving_step.argtypes = [_p, _u32, _u32, _u64, _u32, _u32, _p, _p]
+        L.gal_serving_check.argtypes = [_p, _p, _p, _p]
+        L.gal_serving_table.argtypes = [_p, _p, _p, _p]
+        L.gal_serving_destroy.argtypes = [_p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

@@ -259,3 +259,339 @@ int gal_table_stats(gal_table* t, uint64_t* keys, uint64_t* buckets, uint64_t* l
 }
 
 }  // extern "C"
+
+// ---- serving loop ----
+//
+// Device-resident sessions, each owning a fixed_vector of 64-bit words whose
+// chunks grow from 128 B to 32 KiB (64 KiB in all) on the global allocator.
+// A step is three launches on one stream:
+//   1. session pass: every live session ends with probability
+//      end_q32 / 2^32 (its vector is freed, its slot returned), or appends
+//      tokens x 32 words (tokens drawn from a 1024-entry quantile table)
+//      with one wave per session writing consecutive words;
+//   2. admission: the step's arrivals take free slots if the allocator's
+//      free segments exceed `reserve_segments`, else all are refused;
+//   3. table traffic: ht_ops operations on the extendible hash table, 70 %
+//      insert / 30 % remove of keys uniform in [1, 2^28].
+// Word p of session id holds (id << 32) | p.
+
+using session_vec = fixed_vector<uint64_t, 16, 8192>;
+
+struct serving_session {
+  session_vec vec;
+  uint64_t id;
+  uint32_t alive;
+  uint32_t born;
+};
+
+enum {
+  kSrvAdmitted = 0, kSrvRefused, kSrvEnded, kSrvWords, kSrvWordsRefused, kSrvChunks,
+  kSrvChunkFailed, kSrvChunksFreed, kSrvHtInserted, kSrvHtRemoved, kSrvHtAbsent,
+  kSrvHtFailed, kSrvCount = 16
+};
+
+struct gal_serving {
+  uint32_t max_sessions = 0;
+  uint64_t seed = 0;
+  serving_session* sessions = nullptr;  // device
+  uint32_t* free_slots = nullptr;       // device stack
+  int32_t* free_top = nullptr;          // device
+  uint32_t* queue = nullptr;            // device work queue of the session pass
+  uint32_t* tokens = nullptr;           // device, 1024 entries
+  uint64_t* free_segments = nullptr;    // device
+  table_large* table = nullptr;
+};
+
+namespace {
+
+__device__ __forceinline__ uint64_t mix3(uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t x = a * 0x9E3779B97F4A7C15ULL ^ b;
+  x ^= x >> 31;
+  x *= 0xBF58476D1CE4E5B9ULL;
+  x ^= c + 0x94D049BB133111EBULL;
+  x ^= x >> 29;
+  x *= 0x94D049BB133111EBULL;
+  return x ^ (x >> 32);
+}
+
+__global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t* top, uint32_t n) {
+  const uint64_t i = gallatin::utils::get_tid();
+  if (i < n) {
+    ss[i].alive = 0;
+    ss[i].id = ~0ULL;
+    free_slots[i] = (uint32_t)(n - 1 - i);
+  }
+  if (i == 0) *top = (int32_t)n;
+}
+
+// Session pass: waves take 64 slots at a time from the queue.
+__global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uint32_t n,
+                                                        uint32_t* free_slots, int32_t* top,
+                                                        uint32_t* queue, const uint32_t* tokens,
+                                                        uint64_t seed, uint32_t step,
+                                                        uint32_t end_q32, uint64_t* ctr) {
+  const uint32_t lane = gallatin::wave::lane();
+  uint64_t ended = 0, words = 0, refused = 0, chunks = 0, chunk_fail = 0, freed = 0;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(queue, 64u);
+    base = gallatin::wave::readfirst(base);
+    if (base >= n) break;
+    const uint32_t slot = base + lane;
+    bool alive = slot < n && ss[slot].alive;
+    const uint64_t id = alive ? ss[slot].id : 0;
+    const uint64_t r = mix3(seed, id, step);
+    const bool ends = alive && (uint32_t)r < end_q32;
+    if (ends) {  // free the vector, return the slot
+      serving_session& se = ss[slot];
+      for (uint32_t d = 0; d < session_vec::n_directory; ++d)
+        if (se.vec.directory[d]) ++freed;
+      se.vec.free_vector();
+      se.alive = 0;
+      const int32_t at = atomicAdd(top, 1);
+      free_slots[at] = slot;
+      ++ended;
+    }
+    // appends, one session at a time, all lanes writing
+    uint64_t todo = gallatin::wave::active() & __ballot(alive && !ends);
+    for (; todo; todo &= todo - 1) {
+      const int l = gallatin::wave::first(todo);
+      const uint32_t sl = gallatin::wave::readlane(slot, l);
+      const uint64_t sid = gallatin::wave::readlane(id, l);
+      const uint64_t rr = gallatin::wave::readlane(r, l);
+      session_vec& v = ss[sl].vec;
+      const uint64_t want = (uint64_t)tokens[(rr >> 32) & 1023] * 32;
+      uint64_t at = gallatin::wave::readfirst(v.size);
+      const uint64_t room = at < 8192 ? 8192 - at : 0;
+      const uint64_t k = want < room ? want : room;
+      const uint64_t end = at + k;
+      while (at < end) {
+        const uint32_t d = session_vec::get_directory_index(at);
+        const uint64_t cend = d == 0 ? 16 : (16ULL << d);
+        const uint64_t stop = cend < end ? cend : end;
+        uint64_t c = 0;
+        if (lane == 0) {
+          const bool had = v.directory[d] != nullptr;
+          c = reinterpret_cast<uint64_t>(v.chunk(d));
+          if (!had && c) ++chunks;
+          if (!c) ++chunk_fail;
+        }
+        c = gallatin::wave::readfirst(c);
+        if (!c) break;
+        uint64_t* w = reinterpret_cast<uint64_t*>(c);
+        const uint64_t off = session_vec::get_local_position(at, d);
+        for (uint64_t j = lane; j < stop - at; j += 64) w[off + j] = (sid << 32) | (at + j);
+        if (lane == 0) words += stop - at;
+        at = stop;
+      }
+      // the session's size covers exactly the words written
+      if (lane == 0) {
+        v.size = at;
+        refused += want - (at - (end - k));
+      }
+    }
+  }
+  const uint64_t v[6] = {ended, words, refused, chunks, chunk_fail, freed};
+  const int idx[6] = {kSrvEnded, kSrvWords, kSrvWordsRefused, kSrvChunks, kSrvChunkFailed,
+                      kSrvChunksFreed};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    uint64_t t = v[i];
+    for (int off = 32; off >= 1; off >>= 1) t += gallatin::wave::fetch(t, (int)(lane ^ (uint32_t)off));
+    if (lane == 0 && t) gallatin::utils::atomic_add(&ctr[idx[i]], t);
+  }
+}
+
+// Free segments of the global allocator (bits set in the segment tree).
+__global__ void k_serve_free_segments(uint64_t* out) {
+  const gallatin::allocators::gal_tree& t = global_gallatin->seg_tree;
+  const uint32_t L = t.num_layers - 1;
+  uint64_t c = 0;
+  for (uint64_t w = threadIdx.x; w < t.layer_words[L]; w += blockDim.x)
+    c += __popcll(gallatin::utils::atomic_load(t.words + t.layer_off[L] + w));
+  for (int off = 32; off >= 1; off >>= 1) c += gallatin::wave::fetch(c, (int)(gallatin::wave::lane() ^ (uint32_t)off));
+  if (gallatin::wave::lane() == 0) atomicAdd((unsigned long long*)out, (unsigned long long)c);
+}
+
+__global__ void k_serve_reset(uint32_t* queue, uint64_t* free_segments) {
+  if (threadIdx.x == 0) {
+    *queue = 0;
+    *free_segments = 0;
+  }
+}
+
+__global__ void k_serve_admit(serving_session* ss, uint32_t* free_slots, int32_t* top,
+                              uint32_t arrivals, uint64_t reserve,
+                              const uint64_t* free_segments, uint32_t step, uint64_t* ctr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= arrivals) return;
+  if (*free_segments <= reserve) {
+    atomicAdd((unsigned long long*)&ctr[kSrvRefused], 1ull);
+    return;
+  }
+  const int32_t at = atomicSub(top, 1);
+  if (at <= 0) {
+    atomicAdd(top, 1);
+    atomicAdd((unsigned long long*)&ctr[kSrvRefused], 1ull);
+    return;
+  }
+  const uint32_t slot = free_slots[at - 1];
+  serving_session& se = ss[slot];
+  se.id = ((uint64_t)step << 20) | i;
+  se.born = step;
+  se.vec.init();
+  se.alive = 1;
+  atomicAdd((unsigned long long*)&ctr[kSrvAdmitted], 1ull);
+}
+
+__global__ void k_serve_table(table_large* t, uint64_t seed, uint32_t step, uint32_t n,
+                              uint64_t* ctr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t r = mix3(seed ^ 0x5eedULL, step, i);
+  const uint64_t key = (r & ((1ULL << 28) - 1)) + 1;
+  if ((r >> 40) % 10 < 7) {
+    if (t->insert(key, r))
+      atomicAdd((unsigned long long*)&ctr[kSrvHtInserted], 1ull);
+    else
+      atomicAdd((unsigned long long*)&ctr[kSrvHtFailed], 1ull);
+  } else if (t->remove(key)) {
+    atomicAdd((unsigned long long*)&ctr[kSrvHtRemoved], 1ull);
+  } else {
+    atomicAdd((unsigned long long*)&ctr[kSrvHtAbsent], 1ull);
+  }
+}
+
+__global__ void k_serve_check(serving_session* ss, uint32_t n, unsigned long long* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !ss[i].alive) return;
+  session_vec& v = ss[i].vec;
+  const uint64_t size = v.size < 8192 ? v.size : 8192;
+  uint64_t bad = 0;
+  for (uint64_t p = 0; p < size; ++p) {
+    const uint32_t d = session_vec::get_directory_index(p);
+    if (!v.directory[d]) {
+      bad += 1;
+      continue;
+    }
+    bad += v[p] != ((ss[i].id << 32) | p);
+  }
+  atomicAdd(&out[0], (unsigned long long)bad);
+  atomicAdd(&out[1], 1ull);
+  atomicAdd(&out[2], (unsigned long long)size);
+}
+
+__global__ void k_serve_free_all(serving_session* ss, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !ss[i].alive) return;
+  ss[i].vec.free_vector();
+  ss[i].alive = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gal_serving_create(uint32_t max_sessions, uint64_t seed, const uint32_t* tokens_table,
+                       gal_serving** out) {
+  if (!out || !tokens_table || max_sessions == 0) return fail(GAL_ERR_INVALID, "bad arguments");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  gal_serving* sv = new gal_serving();
+  sv->max_sessions = max_sessions;
+  sv->seed = seed;
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&sv->sessions, sizeof(serving_session) * max_sessions)) != hipSuccess ||
+      (e = hipMalloc(&sv->free_slots, sizeof(uint32_t) * max_sessions)) != hipSuccess ||
+      (e = hipMalloc(&sv->free_top, sizeof(int32_t))) != hipSuccess ||
+      (e = hipMalloc(&sv->queue, sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&sv->free_segments, sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&sv->tokens, sizeof(uint32_t) * 1024)) != hipSuccess ||
+      (e = hipMemcpy(sv->tokens, tokens_table, sizeof(uint32_t) * 1024,
+                     hipMemcpyHostToDevice)) != hipSuccess) {
+    gal_serving_destroy(sv);
+    return hip_fail(e, "serving buffers");
+  }
+  k_serve_init<<<(max_sessions + 255) / 256, 256>>>(sv->sessions, sv->free_slots, sv->free_top,
+                                                    max_sessions);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    gal_serving_destroy(sv);
+    return hip_fail(e, "serving init");
+  }
+  sv->table = table_large::generate_on_device(seed);
+  if (!sv->table) {
+    gal_serving_destroy(sv);
+    return fail(GAL_ERR_HIP, "serving table creation failed");
+  }
+  *out = sv;
+  return GAL_OK;
+}
+
+int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t reserve_segments,
+                     uint32_t end_q32, uint32_t ht_ops, uint64_t* counters_dev, void* stream) {
+  if (!sv || !counters_dev) return fail(GAL_ERR_INVALID, "null argument");
+  if (arrivals >= (1u << 20)) return fail(GAL_ERR_INVALID, "at most 2^20 arrivals per step");
+  hipStream_t st = (hipStream_t)stream;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  k_serve_reset<<<1, 64, 0, st>>>(sv->queue, sv->free_segments);
+  const uint32_t waves = (sv->max_sessions + 63) / 64;
+  uint32_t groups = (uint32_t)cus * 8;
+  if (groups * 4 > waves) groups = (waves + 3) / 4;
+  k_serve_sessions<<<groups, 256, 0, st>>>(sv->sessions, sv->max_sessions, sv->free_slots,
+                                           sv->free_top, sv->queue, sv->tokens, sv->seed, step,
+                                           end_q32, counters_dev);
+  k_serve_free_segments<<<1, 256, 0, st>>>(sv->free_segments);
+  if (arrivals)
+    k_serve_admit<<<(arrivals + 255) / 256, 256, 0, st>>>(sv->sessions, sv->free_slots,
+                                                          sv->free_top, arrivals, reserve_segments,
+                                                          sv->free_segments, step, counters_dev);
+  if (ht_ops)
+    k_serve_table<<<(ht_ops + 255) / 256, 256, 0, st>>>(sv->table, sv->seed, step, ht_ops,
+                                                        counters_dev);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "serving step");
+}
+
+// Live sessions, words checked, mismatching words.  Synchronous.
+int gal_serving_check(gal_serving* sv, uint64_t* live, uint64_t* words, uint64_t* bad) {
+  if (!sv) return fail(GAL_ERR_INVALID, "null argument");
+  unsigned long long* c = nullptr;
+  hipError_t e = hipMalloc(&c, 3 * sizeof(unsigned long long));
+  if (e != hipSuccess) return hip_fail(e, "check buffer");
+  (void)hipMemset(c, 0, 3 * sizeof(unsigned long long));
+  k_serve_check<<<(sv->max_sessions + 255) / 256, 256>>>(sv->sessions, sv->max_sessions, c);
+  unsigned long long h[3] = {0, 0, 0};
+  e = hipMemcpy(h, c, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(c);
+  if (e != hipSuccess) return hip_fail(e, "check copy");
+  if (bad) *bad = h[0];
+  if (live) *live = h[1];
+  if (words) *words = h[2];
+  return GAL_OK;
+}
+
+int gal_serving_table(gal_serving* sv, uint64_t* keys, uint64_t* buckets, uint64_t* failed) {
+  if (!sv || !sv->table) return fail(GAL_ERR_INVALID, "null argument");
+  gal_table t{1, sv->table};
+  return gal_table_stats(&t, keys, buckets, nullptr, failed);
+}
+
+int gal_serving_destroy(gal_serving* sv) {
+  if (!sv) return GAL_OK;
+  if (sv->sessions) {
+    k_serve_free_all<<<(sv->max_sessions + 255) / 256, 256>>>(sv->sessions, sv->max_sessions);
+    (void)hipDeviceSynchronize();
+  }
+  if (sv->table) table_large::free_on_device(sv->table);
+  if (sv->sessions) (void)hipFree(sv->sessions);
+  if (sv->free_slots) (void)hipFree(sv->free_slots);
+  if (sv->free_top) (void)hipFree(sv->free_top);
+  if (sv->queue) (void)hipFree(sv->queue);
+  if (sv->free_segments) (void)hipFree(sv->free_segments);
+  if (sv->tokens) (void)hipFree(sv->tokens);
+  delete sv;
+  return GAL_OK;
+}
+
+}  // extern "C"

@@ -241,6 +241,32 @@ int gal_table_remove(gal_table *t, const uint64_t *keys_dev, uint64_t n,
 int gal_table_stats(gal_table *t, uint64_t *keys, uint64_t *buckets,
                     uint64_t *level, uint64_t *failed);
 
+/* ---- serving loop (no reference counterpart; on the global allocator) ----
+ * Device-resident sessions, each owning a fixed_vector<uint64_t, 16, 8192>
+ * (128 B .. 64 KiB) and a shared gal_table of kind 1.  A step (three
+ * launches on `stream`): each live session ends with probability
+ * end_q32 / 2^32 (vector freed) or appends tokens x 32 words, tokens drawn
+ * from tokens_table[1024]; then `arrivals` new sessions are admitted if the
+ * allocator has more than reserve_segments free segments (else refused);
+ * then ht_ops table operations, 70 % insert / 30 % remove, keys uniform in
+ * [1, 2^28].  Counters (device, 16 x u64): [0] admitted, [1] refused, [2]
+ * ended, [3] words appended, [4] words refused (vector full), [5] chunks
+ * allocated, [6] chunk allocations failed, [7] chunks freed, [8] table
+ * inserts, [9] removes that found the key, [10] removes of absent keys,
+ * [11] inserts refused. */
+typedef struct gal_serving gal_serving;
+int gal_serving_create(uint32_t max_sessions, uint64_t seed,
+                       const uint32_t *tokens_table_host, gal_serving **out);
+int gal_serving_step(gal_serving *s, uint32_t step, uint32_t arrivals,
+                     uint64_t reserve_segments, uint32_t end_q32, uint32_t ht_ops,
+                     uint64_t *counters_dev, void *stream);
+/* Live sessions, words checked, words not equal to (id << 32) | position. */
+int gal_serving_check(gal_serving *s, uint64_t *live, uint64_t *words, uint64_t *bad);
+/* Keys, buckets and refused inserts of the session table. */
+int gal_serving_table(gal_serving *s, uint64_t *keys, uint64_t *buckets, uint64_t *failed);
+/* Frees every live session's vector and the table. */
+int gal_serving_destroy(gal_serving *s);
+
 /* ---- batched request streams (no reference counterpart) ----
  * A stream is nbatches batches of ops; batch b is ops[offsets[b] ..
  * offsets[b+1]) (offsets is a host array of nbatches + 1 entries).  kind 0:
