@@ -57,6 +57,10 @@ def parse():
                    help="skip the batched Zipf-stream sweep")
     p.add_argument("--zipf-pool-gib", type=int, default=64)
     p.add_argument("--zipf-reps", type=int, default=3)
+    p.add_argument("--no-serving", action="store_true", help="skip the serving loop")
+    p.add_argument("--serving-pool-gib", type=int, default=200)
+    p.add_argument("--serving-steps", type=int, default=200)
+    p.add_argument("--serving-warmup", type=int, default=300)
     return p.parse_args()
 
 
@@ -159,6 +163,29 @@ def zipf_sweep(args, G, torch):
                   for r in table],
     }
     return out
+
+
+def serving_run(args, G, torch):
+    """Serving form: 200 GiB global allocator, Poisson(2000) arrivals per
+    step, geometric(0.01) lifetimes, lognormal(3, 1) tokens x 256 B appended
+    per session per step to a 128 B .. 64 KiB fixed_vector, admission gated
+    on 5 % free segments, 2^20 hash-table operations per step (70 % insert /
+    30 % remove, keys uniform in [1, 2^28])."""
+    from gallatin_amd import serving
+    G.init_global_allocator(args.serving_pool_gib << 30, args.seed, print_info=False)
+    try:
+        r = serving.run(torch, steps=args.serving_steps, warmup=args.serving_warmup,
+                        seed=args.seed)
+        g = G.global_allocator()
+        r["allocator_errors"] = g.info()["errors"]
+    finally:
+        G.free_global_allocator()
+    r["workload"] = ("200 GiB pool, Poisson(2000) arrivals/step, geometric(0.01) lifetime, "
+                     "lognormal(3,1) tokens x 256 B per session-step into 128 B..64 KiB "
+                     "vector chunks, 5% free-segment admission reserve, 2^20 table ops/step "
+                     "(70% insert / 30% remove, keys in [1, 2^28])")
+    r["pool_bytes"] = args.serving_pool_gib << 30
+    return r
 
 
 def cpu_baselines(args):
@@ -315,6 +342,8 @@ def main():
         }
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)
+    if not args.no_serving:
+        out["serving"] = serving_run(args, G, torch)
     if not args.no_cpu_baseline:
         out.update(cpu_baselines(args))
     print(json.dumps(out))
