@@ -49,6 +49,7 @@ CAPI_SYMBOLS = [
     "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
     "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
     "gal_serving_check", "gal_serving_table", "gal_serving_destroy",
+    "gal_queue_create", "gal_queue_destroy", "gal_queue_enqueue", "gal_queue_dequeue",
 ]
 
 
@@ -139,6 +140,10 @@ def lib():
         L.gal_serving_check.argtypes = [_p, _p, _p, _p]
         L.gal_serving_table.argtypes = [_p, _p, _p, _p]
         L.gal_serving_destroy.argtypes = [_p]
+        L.gal_queue_create.argtypes = [_p, ctypes.c_int, ctypes.POINTER(_p)]
+        L.gal_queue_destroy.argtypes = [_p]
+        L.gal_queue_enqueue.argtypes = [_p, _p, _u64, _p, ctypes.c_int, _p]
+        L.gal_queue_dequeue.argtypes = [_p, _p, _p, _u64, ctypes.c_int, _p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -443,6 +448,35 @@ class Table:
     def destroy(self):
         if self.handle:
             _check(lib().gal_table_destroy(self.handle))
+            self.handle = _p()
+
+
+class Queue:
+    """queue<uint64_t, Gallatin> (queue.cuh:59-192) on an allocator; ring=True
+    for circular_CAS_queue<uint64_t, 8, ...> (queue.cuh:196-253)."""
+
+    def __init__(self, alloc, ring=False):
+        self.handle = _p()
+        _check(lib().gal_queue_create(alloc.handle, int(bool(ring)), ctypes.byref(self.handle)))
+
+    def enqueue(self, items, serial=False):
+        import torch
+        ok = torch.zeros(items.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_queue_enqueue(self.handle, items.data_ptr(), items.numel(),
+                                       ok.data_ptr(), int(bool(serial)), _stream()))
+        return ok
+
+    def dequeue(self, n, serial=False):
+        import torch
+        out = torch.zeros(n, dtype=torch.int64, device="cuda")
+        ok = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        _check(lib().gal_queue_dequeue(self.handle, out.data_ptr(), ok.data_ptr(), n,
+                                       int(bool(serial)), _stream()))
+        return out, ok
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().gal_queue_destroy(self.handle))
             self.handle = _p()
 
 

@@ -9,6 +9,7 @@
 
 #include "gallatin/allocators/batched.hpp"
 #include "gallatin/allocators/gallatin.hpp"
+#include "gallatin/data_structs/queue.hpp"
 #include "gallatin/streams.h"
 #include "gallatin_capi.h"
 #include "internal.hpp"
@@ -411,6 +412,112 @@ int gal_bench_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
   k_bench_copy<<<(unsigned)cus * 8, 256, 0, (hipStream_t)stream>>>(
       static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
   return launch_status("bench_copy");
+}
+
+}  // extern "C"
+
+// ---- allocator-backed queues ----
+//
+// The device methods of Gallatin<> read the instance's geometry at run time,
+// so one instantiation serves every gal_allocator.
+using queue_alloc = Gallatin<16ULL << 20, 16, 4096>;
+using queue_one = gallatin::data_structs::queue<uint64_t, queue_alloc>;
+using queue_ring = gallatin::data_structs::circular_CAS_queue<uint64_t, 8, queue_alloc>;
+
+struct gal_queue {
+  int ring = 0;
+  void* dev = nullptr;
+};
+
+namespace {
+
+template <typename Q>
+__global__ void k_queue_enqueue(Q* q, const uint64_t* items, uint64_t n, uint8_t* ok,
+                                int serial) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (serial) {  // one thread, in order
+    if (i == 0)
+      for (uint64_t k = 0; k < n; ++k) {
+        const bool r = q->try_enqueue(items[k]);
+        if (ok) ok[k] = r;
+      }
+    return;
+  }
+  if (i >= n) return;
+  const bool r = q->try_enqueue(items[i]);
+  if (ok) ok[i] = r;
+}
+
+template <typename Q>
+__global__ void k_queue_dequeue(Q* q, uint64_t* out, uint8_t* ok, uint64_t n, int serial) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (serial) {
+    if (i == 0)
+      for (uint64_t k = 0; k < n; ++k) {
+        uint64_t v = 0;
+        const bool r = q->dequeue(v);
+        out[k] = r ? v : 0;
+        if (ok) ok[k] = r;
+      }
+    return;
+  }
+  if (i >= n) return;
+  uint64_t v = 0;
+  const bool r = q->dequeue(v);
+  out[i] = r ? v : 0;
+  if (ok) ok[i] = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gal_queue_create(gal_allocator* a, int ring, gal_queue** out) {
+  if (!a || !out) return fail(GAL_ERR_INVALID, "null argument");
+  queue_alloc* al = static_cast<queue_alloc*>(static_cast<gal_state*>(a->dev));
+  void* dev = ring ? (void*)queue_ring::generate_on_device(al)
+                   : (void*)queue_one::generate_on_device(al);
+  if (!dev) return fail(GAL_ERR_HIP, "queue creation failed");
+  *out = new gal_queue{ring ? 1 : 0, dev};
+  return GAL_OK;
+}
+
+int gal_queue_destroy(gal_queue* q) {
+  if (!q) return GAL_OK;
+  if (q->ring)
+    queue_ring::free_on_device(static_cast<queue_ring*>(q->dev));
+  else
+    queue_one::free_on_device(static_cast<queue_one*>(q->dev));
+  delete q;
+  return GAL_OK;
+}
+
+int gal_queue_enqueue(gal_queue* q, const uint64_t* items, uint64_t n, uint8_t* ok, int serial,
+                      void* stream) {
+  if (!q || (!items && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  const unsigned g = serial ? 1u : grid_for(n, 256);
+  if (q->ring)
+    k_queue_enqueue<<<g, 256, 0, (hipStream_t)stream>>>(static_cast<queue_ring*>(q->dev), items,
+                                                        n, ok, serial);
+  else
+    k_queue_enqueue<<<g, 256, 0, (hipStream_t)stream>>>(static_cast<queue_one*>(q->dev), items, n,
+                                                        ok, serial);
+  return launch_status("queue enqueue");
+}
+
+int gal_queue_dequeue(gal_queue* q, uint64_t* out, uint8_t* ok, uint64_t n, int serial,
+                      void* stream) {
+  if (!q || (!out && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  const unsigned g = serial ? 1u : grid_for(n, 256);
+  if (q->ring)
+    k_queue_dequeue<<<g, 256, 0, (hipStream_t)stream>>>(static_cast<queue_ring*>(q->dev), out, ok,
+                                                        n, serial);
+  else
+    k_queue_dequeue<<<g, 256, 0, (hipStream_t)stream>>>(static_cast<queue_one*>(q->dev), out, ok,
+                                                        n, serial);
+  return launch_status("queue dequeue");
 }
 
 }  // extern "C"

@@ -241,6 +241,20 @@ int gal_table_remove(gal_table *t, const uint64_t *keys_dev, uint64_t n,
 int gal_table_stats(gal_table *t, uint64_t *keys, uint64_t *buckets,
                     uint64_t *level, uint64_t *failed);
 
+/* gal_queue: queue<uint64_t, Gallatin<...>> (data_structs/queue.cuh:59-192,
+ * header include/gallatin/data_structs/queue.hpp) on allocator `a`, or with
+ * ring != 0 the ring of 8 such queues (circular_CAS_queue, queue.cuh:196).
+ * enqueue: ok[i] = 0 when no node could be allocated.  dequeue: ok[i] = 0
+ * when the (chosen) queue was empty.  serial != 0: one thread does all n
+ * operations in order (FIFO checks).  destroy frees every node left. */
+typedef struct gal_queue gal_queue;
+int gal_queue_create(gal_allocator *a, int ring, gal_queue **out);
+int gal_queue_destroy(gal_queue *q);
+int gal_queue_enqueue(gal_queue *q, const uint64_t *items_dev, uint64_t n,
+                      uint8_t *ok_dev, int serial, void *stream);
+int gal_queue_dequeue(gal_queue *q, uint64_t *out_dev, uint8_t *ok_dev, uint64_t n,
+                      int serial, void *stream);
+
 /* ---- serving loop (no reference counterpart; on the global allocator) ----
  * Device-resident sessions, each owning a fixed_vector<uint64_t, 16, 8192>
  * (128 B .. 64 KiB) and a shared gal_table of kind 1.  A step (three
