@@ -123,6 +123,34 @@ def atomic_ceiling(G, torch):
     return out
 
 
+def plain_small(args, G, torch):
+    """Plain method, small: 1 GiB pool, Gallatin<16 MiB, 16, 4096>, one
+    launch of 2^20 threads each doing malloc(16), write its uint32 id, read
+    it back, free; 10 launches per step."""
+    a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(1 << 30, args.seed, print_info=False)
+    ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
+    n, launches, steps = 1 << 20, 10, 5
+    for _ in range(launches):  # warm-up step
+        a.run_plain(n, 16, counters=ctr)
+    torch.cuda.synchronize()
+    ctr.zero_()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps * launches):
+        a.run_plain(n, 16, counters=ctr)
+    e1.record()
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    info = a.info()
+    a.free_on_device()
+    secs = e0.elapsed_time(e1) / 1e3
+    return {"workload": "1 GiB pool, 2^20 threads x malloc(16)/write id/read/free, "
+                        "10 launches per step",
+            "value": c[G.CTR_PAIRS] / secs, "unit": "pairs/s",
+            "ms_per_step": secs * 1e3 / steps, "misses": c[G.CTR_MISSES],
+            "violations": c[G.CTR_BAD] + info["errors"]}
+
+
 def zipf_sweep(args, G, torch):
     """Batched persistent form: 256 batches with Zipf(1.2) request counts on
     [1, 2^20], 60 % malloc of lognormal(6, 2) bytes clipped to [1, 16 MiB] /
@@ -340,6 +368,7 @@ def main():
             "achieved": atoms, "peak": ceil["per_line"], "unit": "atomics/s",
             "frac": atoms / ceil["per_line"], "per_pair": 2, "ceilings": ceil,
         }
+    out["plain_small"] = plain_small(args, G, torch)
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)
     if not args.no_serving:

@@ -23,6 +23,7 @@ def kernel_key(name):
     name = name.strip()
     if name.startswith("void "):
         name = name[5:]
+    name = name.replace("(anonymous namespace)::", "")
     depth = 0
     for i, ch in enumerate(name):
         if ch == "<":

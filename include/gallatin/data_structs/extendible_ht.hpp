@@ -203,6 +203,22 @@ struct extendible_hash_table {
 
   __device__ uint64_t hash(Key k) const { return hashers::MurmurHash64A_u64((uint64_t)k, seed); }
 
+  // All keys of a bucket are loaded first (the loads are in flight
+  // together), then compared: the slot holding `key` or -1, and the first
+  // free slot or -1.
+  __device__ void scan(bucket* b, uint64_t key, int& match, int& empty) {
+    uint64_t k[items_per_bucket];
+#pragma unroll
+    for (int s = 0; s < items_per_bucket; ++s) k[s] = ld(&b->keys[s]);
+    match = -1;
+    empty = -1;
+#pragma unroll
+    for (int s = items_per_bucket - 1; s >= 0; --s) {
+      if (k[s] == key) match = s;
+      if (k[s] == (uint64_t)defaultKey || k[s] == (uint64_t)tombstoneKey) empty = s;
+    }
+  }
+
   // The bucket owning hash h: the entry of its low `level` bits, or of
   // fewer bits when that home has no bucket yet.
   __device__ bucket* owner(uint64_t h) {
@@ -284,17 +300,12 @@ struct extendible_hash_table {
         continue;
       }
       if (owns(b, h)) {
-        int free_slot = -1;
-        for (int s = 0; s < items_per_bucket && state == 0; ++s) {
-          const uint64_t k = ld(&b->keys[s]);
-          if (k == (uint64_t)key) {
-            st(&b->vals[s], (uint64_t)val);
-            state = 1;
-          } else if (free_slot < 0 && (k == defaultKey || k == tombstoneKey)) {
-            free_slot = s;
-          }
-        }
-        if (state == 0 && free_slot >= 0) {
+        int match, free_slot;
+        scan(b, (uint64_t)key, match, free_slot);
+        if (match >= 0) {
+          st(&b->vals[match], (uint64_t)val);
+          state = 1;
+        } else if (free_slot >= 0) {
           st(&b->vals[free_slot], (uint64_t)val);
           __builtin_amdgcn_s_waitcnt(0);  // value before key
           st(&b->keys[free_slot], (uint64_t)key);
@@ -319,11 +330,11 @@ struct extendible_hash_table {
       if (!b) return false;
       const uint32_t d0 = gallatin::utils::atomic_load(&b->depth);
       if ((h & mask(d0)) != ld(&b->home)) continue;  // being split: look again
-      for (int s = 0; s < items_per_bucket; ++s) {
-        if (ld(&b->keys[s]) == (uint64_t)key) {
-          val = (Val)ld(&b->vals[s]);
-          return true;
-        }
+      int match, empty;
+      scan(b, (uint64_t)key, match, empty);
+      if (match >= 0) {
+        val = (Val)ld(&b->vals[match]);
+        return true;
       }
       if (gallatin::utils::atomic_load(&b->depth) == d0) return false;
     }
@@ -342,13 +353,12 @@ struct extendible_hash_table {
         continue;
       }
       if (owns(b, h)) {
+        int match, empty;
+        scan(b, (uint64_t)key, match, empty);
         state = 2;
-        for (int s = 0; s < items_per_bucket; ++s) {
-          if (ld(&b->keys[s]) == (uint64_t)key) {
-            st(&b->keys[s], (uint64_t)defaultKey);
-            state = 1;
-            break;
-          }
+        if (match >= 0) {
+          st(&b->keys[match], (uint64_t)defaultKey);
+          state = 1;
         }
       }
       unlock(b);
