@@ -304,6 +304,16 @@ struct gal_serving {
 
 namespace {
 
+// Adds the number of active lanes with `pred` set to *c with one atomic per
+// wave.  Every active lane must call it.  Per-lane atomics on one shared
+// counter serialise at one memory channel (about 88 M/s on MI355X), which
+// would dominate a kernel of a million short operations.
+__device__ __forceinline__ void count_wave(uint64_t* c, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (m && gallatin::wave::is_first())
+    gallatin::utils::atomic_add(c, (uint64_t)gallatin::wave::count(m));
+}
+
 __device__ __forceinline__ uint64_t mix3(uint64_t a, uint64_t b, uint64_t c) {
   uint64_t x = a * 0x9E3779B97F4A7C15ULL ^ b;
   x ^= x >> 31;
@@ -342,14 +352,18 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
     const uint64_t id = alive ? ss[slot].id : 0;
     const uint64_t r = mix3(seed, id, step);
     const bool ends = alive && (uint32_t)r < end_q32;
+    // slots of ending sessions go back on the stack, one push per wave
+    const uint64_t em = __ballot(ends);
+    int32_t at0 = 0;
+    if (em && gallatin::wave::is_first()) at0 = atomicAdd(top, (int32_t)gallatin::wave::count(em));
+    at0 = (int32_t)gallatin::wave::readfirst((uint32_t)at0);
     if (ends) {  // free the vector, return the slot
       serving_session& se = ss[slot];
       for (uint32_t d = 0; d < session_vec::n_directory; ++d)
         if (se.vec.directory[d]) ++freed;
       se.vec.free_vector();
       se.alive = 0;
-      const int32_t at = atomicAdd(top, 1);
-      free_slots[at] = slot;
+      free_slots[at0 + (int32_t)gallatin::wave::count(em & gallatin::wave::lanes_below())] = slot;
       ++ended;
     }
     // appends, one session at a time, all lanes writing
@@ -425,23 +439,31 @@ __global__ void k_serve_admit(serving_session* ss, uint32_t* free_slots, int32_t
                               const uint64_t* free_segments, uint32_t step, uint64_t* ctr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= arrivals) return;
-  if (*free_segments <= reserve) {
-    atomicAdd((unsigned long long*)&ctr[kSrvRefused], 1ull);
-    return;
+  // One pop of the free-slot stack per wave: the wave takes as many slots
+  // as it has arrivals, and the lanes past what the stack held give theirs
+  // back.
+  const bool room = *free_segments > reserve;
+  const uint64_t want = __ballot(room);
+  const uint32_t rank = gallatin::wave::count(want & gallatin::wave::lanes_below());
+  int32_t top0 = 0;
+  if (want && gallatin::wave::is_first()) {
+    const int32_t n = (int32_t)gallatin::wave::count(want);
+    top0 = atomicSub(top, n);
+    const int32_t got = top0 < 0 ? 0 : (top0 < n ? top0 : n);
+    if (got < n) atomicAdd(top, n - got);
   }
-  const int32_t at = atomicSub(top, 1);
-  if (at <= 0) {
-    atomicAdd(top, 1);
-    atomicAdd((unsigned long long*)&ctr[kSrvRefused], 1ull);
-    return;
+  top0 = (int32_t)gallatin::wave::readfirst((uint32_t)top0);
+  const bool admitted = room && (int32_t)rank < top0;
+  if (admitted) {
+    const uint32_t slot = free_slots[top0 - 1 - (int32_t)rank];
+    serving_session& se = ss[slot];
+    se.id = ((uint64_t)step << 20) | i;
+    se.born = step;
+    se.vec.init();
+    se.alive = 1;
   }
-  const uint32_t slot = free_slots[at - 1];
-  serving_session& se = ss[slot];
-  se.id = ((uint64_t)step << 20) | i;
-  se.born = step;
-  se.vec.init();
-  se.alive = 1;
-  atomicAdd((unsigned long long*)&ctr[kSrvAdmitted], 1ull);
+  count_wave(&ctr[kSrvAdmitted], admitted);
+  count_wave(&ctr[kSrvRefused], !admitted);
 }
 
 __global__ void k_serve_table(table_large* t, uint64_t seed, uint32_t step, uint32_t n,
@@ -450,16 +472,12 @@ __global__ void k_serve_table(table_large* t, uint64_t seed, uint32_t step, uint
   if (i >= n) return;
   const uint64_t r = mix3(seed ^ 0x5eedULL, step, i);
   const uint64_t key = (r & ((1ULL << 28) - 1)) + 1;
-  if ((r >> 40) % 10 < 7) {
-    if (t->insert(key, r))
-      atomicAdd((unsigned long long*)&ctr[kSrvHtInserted], 1ull);
-    else
-      atomicAdd((unsigned long long*)&ctr[kSrvHtFailed], 1ull);
-  } else if (t->remove(key)) {
-    atomicAdd((unsigned long long*)&ctr[kSrvHtRemoved], 1ull);
-  } else {
-    atomicAdd((unsigned long long*)&ctr[kSrvHtAbsent], 1ull);
-  }
+  const bool ins = (r >> 40) % 10 < 7;
+  const bool ok = ins ? t->insert(key, r) : t->remove(key);
+  count_wave(&ctr[kSrvHtInserted], ins && ok);
+  count_wave(&ctr[kSrvHtFailed], ins && !ok);
+  count_wave(&ctr[kSrvHtRemoved], !ins && ok);
+  count_wave(&ctr[kSrvHtAbsent], !ins && !ok);
 }
 
 __global__ void k_serve_check(serving_session* ss, uint32_t n, unsigned long long* out) {

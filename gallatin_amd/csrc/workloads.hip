@@ -297,9 +297,31 @@ __global__ void k_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,
   if (acc == 0xFFFFFFFFu) words[0] = acc;
 }
 
-__global__ void k_bench_copy(const uint4* src, uint4* dst, uint64_t n) {
+// Each workgroup copies whole tiles of kCopyVecs * blockDim.x 16-byte
+// vectors (a contiguous 64 KiB at 512 threads), tiles dealt round robin;
+// the vectors past the last whole tile go grid-stride.  Contiguous
+// per-workgroup tiles measured about 12 % faster than a pure grid-stride
+// copy on MI355X.
+constexpr int kCopyVecs = 8;
+constexpr int kCopyThreads = 512;
+
+__global__ void __launch_bounds__(kCopyThreads)
+    k_bench_copy(const u64x2* src_, u64x2* dst_, uint64_t n) {
+  const global_u64x2* src = (const global_u64x2*)src_;
+  global_u64x2* dst = (global_u64x2*)dst_;
+  const uint64_t tile = (uint64_t)blockDim.x * kCopyVecs;
+  const uint64_t tiles = n / tile;
+  for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const uint64_t b = t * tile + threadIdx.x;
+    u64x2 v[kCopyVecs];
+#pragma unroll
+    for (int u = 0; u < kCopyVecs; ++u) v[u] = src[b + (uint64_t)u * blockDim.x];
+#pragma unroll
+    for (int u = 0; u < kCopyVecs; ++u) dst[b + (uint64_t)u * blockDim.x] = v[u];
+  }
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nthreads)
+  for (uint64_t i = tiles * tile + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += nthreads)
     dst[i] = src[i];
 }
 
@@ -409,8 +431,8 @@ int gal_bench_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  k_bench_copy<<<(unsigned)cus * 8, 256, 0, (hipStream_t)stream>>>(
-      static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
+  k_bench_copy<<<(unsigned)cus * 8, kCopyThreads, 0, (hipStream_t)stream>>>(
+      static_cast<const u64x2*>(src), static_cast<u64x2*>(dst), bytes / 16);
   return launch_status("bench_copy");
 }
 

@@ -60,6 +60,10 @@ __device__ __forceinline__ T atomic_and(T* p, T v) {
   return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class T>
+__device__ __forceinline__ T atomic_max(T* p, T v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
 __device__ __forceinline__ T atomic_exch(T* p, T v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

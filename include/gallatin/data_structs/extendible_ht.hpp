@@ -9,20 +9,24 @@
 // that home exists yet.  The owner of a hash is found by looking at the
 // entry of its low `level` bits, then of its low level - 1 bits, ... and
 // taking the first bucket present (the reference's refinement from the
-// global level down, extendible_ht.cuh:842-1115).  Entries [0, 2^min_bits)
-// are filled at creation.
+// global level down, extendible_ht.cuh:842-1115); the entries of four
+// depths are loaded together.  Entries [0, 2^min_bits) are filled at
+// creation.
 //
 // Writes (insert, remove) lock the owner bucket, check that it still owns
-// the key and work under the lock.  A full bucket is split: a new bucket
-// with home h(b) + 2^d(b) and depth d(b) + 1 receives the keys whose bit
-// d(b) is set, is published in the directory, then the old bucket's depth
-// is raised and the moved keys are cleared from it (the reference's
-// promotion, extendible_ht.cuh:1118-1289).  Queries take no lock: they
-// read the owner's depth, scan its keys, and when the key is missing look
-// again if the depth changed meanwhile, so a key is never missed while it
-// moves.  Every bucket word is read and written with device-coherent
-// atomics; a value is written before its key and a moved key before the
-// new bucket is published.  Lock attempts per operation are bounded.
+// the key and work under the lock.  The lock bit and the depth share one
+// word, so the atomic that takes the lock also returns the depth.  A full
+// bucket is split: a new bucket with home h(b) + 2^d(b) and depth d(b) + 1
+// receives the keys whose bit d(b) is set, is published in the directory,
+// then the old bucket's depth is raised and the moved keys are cleared from
+// it (the reference's promotion, extendible_ht.cuh:1118-1289).  Queries
+// take no lock: they read the owner's depth, scan its keys, and when the key
+// is missing look again if the depth changed meanwhile, so a key is never
+// missed while it moves; a found key's value counts only if the key is still
+// in its slot after the value was read (the slot may have been emptied and
+// reused).  Every bucket word is read and written with device-coherent
+// atomics; a value is written before its key and a moved key before the new
+// bucket is published.  Lock attempts per operation are bounded.
 //
 // Differences from the reference: insert of a present key replaces its
 // value; remove() exists (the reference has none, so its behaviour is not
@@ -84,8 +88,7 @@ struct extendible_hash_table {
   static constexpr uint32_t kTries = 1u << 22;  // lock attempts per operation
 
   struct bucket {
-    uint32_t lock;
-    uint32_t depth;
+    uint64_t ctl;  // (depth << 32) | lock bit
     uint64_t home;
     uint64_t keys[items_per_bucket];
     uint64_t vals[items_per_bucket];
@@ -154,8 +157,7 @@ struct extendible_hash_table {
     if (!b) return nullptr;
     // device-coherent stores: another XCD may read the bucket as soon as
     // it is published
-    gallatin::utils::atomic_store(&b->lock, 0u);
-    gallatin::utils::atomic_store(&b->depth, depth);
+    gallatin::utils::atomic_store(&b->ctl, (uint64_t)depth << 32);
     gallatin::utils::atomic_store(&b->home, home);
     for (int i = 0; i < items_per_bucket; ++i)
       gallatin::utils::atomic_store(&b->keys[i], (uint64_t)defaultKey);
@@ -203,11 +205,12 @@ struct extendible_hash_table {
 
   __device__ uint64_t hash(Key k) const { return hashers::MurmurHash64A_u64((uint64_t)k, seed); }
 
-  // All keys of a bucket are loaded first (the loads are in flight
-  // together), then compared: the slot holding `key` or -1, and the first
-  // free slot or -1.
-  __device__ void scan(bucket* b, uint64_t key, int& match, int& empty) {
+  // All keys of a bucket (and its home) are loaded first, so the loads are
+  // in flight together, then compared: the slot holding `key` or -1, and
+  // the first free slot or -1.
+  __device__ uint64_t scan(bucket* b, uint64_t key, int& match, int& empty) {
     uint64_t k[items_per_bucket];
+    const uint64_t home = ld(&b->home);
 #pragma unroll
     for (int s = 0; s < items_per_bucket; ++s) k[s] = ld(&b->keys[s]);
     match = -1;
@@ -217,66 +220,82 @@ struct extendible_hash_table {
       if (k[s] == key) match = s;
       if (k[s] == (uint64_t)defaultKey || k[s] == (uint64_t)tombstoneKey) empty = s;
     }
+    return home;
   }
 
   // The bucket owning hash h: the entry of its low `level` bits, or of
-  // fewer bits when that home has no bucket yet.
+  // fewer bits when that home has no bucket yet.  Entries are only ever
+  // set, so the deepest one present among the loaded depths is the owner
+  // (or a bucket being split, which the callers' home check rejects).
+  static constexpr int kOwnerFan = 4;
   __device__ bucket* owner(uint64_t h) {
-    const uint64_t top = ld(&level);
-    for (uint64_t d = top; d >= min_bits; --d) {
-      const uint64_t e = ld(reinterpret_cast<uint64_t*>(&directory[h & mask(d)]));
-      if (e) return reinterpret_cast<bucket*>(e);
-      if (d == min_bits) break;
+    uint64_t top = ld(&level);
+    for (;;) {
+      uint64_t e[kOwnerFan];
+#pragma unroll
+      for (int j = 0; j < kOwnerFan; ++j) {
+        const uint64_t d = top >= min_bits + j ? top - j : min_bits;
+        e[j] = ld(reinterpret_cast<uint64_t*>(&directory[h & mask(d)]));
+      }
+#pragma unroll
+      for (int j = 0; j < kOwnerFan; ++j)
+        if (e[j]) return reinterpret_cast<bucket*>(e[j]);
+      if (top < min_bits + kOwnerFan) return nullptr;
+      top -= kOwnerFan;
     }
-    return nullptr;
   }
 
-  // One attempt.  Lock users retry in a loop whose every iteration holds
-  // the whole critical section: a lane that gets the lock releases it in
-  // the same iteration, so lanes of one wave waiting for the same bucket
-  // never keep its holder from running (a spin-then-work loop would).
-  __device__ bool try_lock(bucket* b) {
-    return gallatin::utils::atomic_load(&b->lock) == 0u &&
-           gallatin::utils::atomic_cas(&b->lock, 0u, 1u) == 0u;
+  static __device__ __forceinline__ uint32_t depth_of(uint64_t ctl) {
+    return (uint32_t)(ctl >> 32);
   }
 
-  __device__ void unlock(bucket* b) {
-    __builtin_amdgcn_s_waitcnt(0);
-    gallatin::utils::atomic_store(&b->lock, 0u);
+  // One attempt; on success `d` is the bucket's depth.  Lock users retry in
+  // a loop whose every iteration holds the whole critical section: a lane
+  // that gets the lock releases it in the same iteration, so lanes of one
+  // wave waiting for the same bucket never keep its holder from running (a
+  // spin-then-work loop would).
+  __device__ bool try_lock(bucket* b, uint32_t& d) {
+    const uint64_t old = gallatin::utils::atomic_or(&b->ctl, (uint64_t)1);
+    d = depth_of(old);
+    return (old & 1) == 0;
   }
 
-  __device__ bool owns(bucket* b, uint64_t h) {
-    const uint32_t d = gallatin::utils::atomic_load(&b->depth);
-    return (h & mask(d)) == ld(&b->home);
+  __device__ void unlock(bucket* b, uint32_t d) {
+    __builtin_amdgcn_s_waitcnt(0);  // the critical section's stores land first
+    gallatin::utils::atomic_store(&b->ctl, (uint64_t)d << 32);
   }
 
   // Split full bucket b (locked, depth d): keys with bit d set move to a new
   // bucket with home h(b) + 2^d.  False when no bucket could be had.
-  __device__ bool split(bucket* b, uint32_t d) {
-    const uint64_t home = ld(&b->home) | (1ULL << d);
-    bucket* nb = new_bucket(home, d + 1);
-    if (!nb) return false;
-    uint64_t moved = 0;
+  __device__ bool split(bucket* b, uint32_t d, uint64_t old_home) {
+    uint64_t k[items_per_bucket], v[items_per_bucket];
+#pragma unroll
     for (int s = 0; s < items_per_bucket; ++s) {
-      const uint64_t k = ld(&b->keys[s]);
-      if (k == defaultKey || k == tombstoneKey) continue;
-      if ((hash((Key)k) >> d) & 1) {
-        st(&nb->vals[s], ld(&b->vals[s]));
-        st(&nb->keys[s], k);
-        moved |= 1ULL << s;
-      }
+      k[s] = ld(&b->keys[s]);
+      v[s] = ld(&b->vals[s]);
     }
-    __builtin_amdgcn_s_waitcnt(0);  // moved keys land before the bucket is published
+    const uint64_t home = old_home | (1ULL << d);
+    bucket* nb = static_cast<bucket*>(gallatin::allocators::global_malloc(sizeof(bucket)));
+    if (!nb) return false;
+    gallatin::utils::atomic_add(&buckets, (uint64_t)1);
+    uint64_t moved = 0;
+#pragma unroll
+    for (int s = 0; s < items_per_bucket; ++s) {
+      const bool live = k[s] != (uint64_t)defaultKey && k[s] != (uint64_t)tombstoneKey;
+      const bool go = live && ((hash((Key)k[s]) >> d) & 1);
+      moved |= (uint64_t)go << s;
+      if (go) st(&nb->vals[s], v[s]);
+      st(&nb->keys[s], go ? k[s] : (uint64_t)defaultKey);
+    }
+    st(&nb->ctl, (uint64_t)(d + 1) << 32);
+    st(&nb->home, home);
+    __builtin_amdgcn_s_waitcnt(0);  // the new bucket is whole before it is published
     st(reinterpret_cast<uint64_t*>(&directory[home]), reinterpret_cast<uint64_t>(nb));
-    uint64_t cur = ld(&level);
-    while (cur < d + 1) {
-      const uint64_t seen = gallatin::utils::atomic_cas(&level, cur, (uint64_t)(d + 1));
-      if (seen == cur) break;
-      cur = seen;
-    }
+    gallatin::utils::atomic_max(&level, (uint64_t)(d + 1));
     __builtin_amdgcn_s_waitcnt(0);
-    gallatin::utils::atomic_store(&b->depth, d + 1);
+    st(&b->ctl, ((uint64_t)(d + 1) << 32) | 1);  // still locked
     __builtin_amdgcn_s_waitcnt(0);  // new depth visible before keys leave
+#pragma unroll
     for (int s = 0; s < items_per_bucket; ++s)
       if ((moved >> s) & 1) st(&b->keys[s], (uint64_t)defaultKey);
     return true;
@@ -295,13 +314,14 @@ struct extendible_hash_table {
         state = 2;
         break;
       }
-      if (!try_lock(b)) {
+      uint32_t d;
+      if (!try_lock(b, d)) {
         __builtin_amdgcn_s_sleep(2);
         continue;
       }
-      if (owns(b, h)) {
-        int match, free_slot;
-        scan(b, (uint64_t)key, match, free_slot);
+      int match, free_slot;
+      const uint64_t home = scan(b, (uint64_t)key, match, free_slot);
+      if ((h & mask(d)) == home) {  // still the owner
         if (match >= 0) {
           st(&b->vals[match], (uint64_t)val);
           state = 1;
@@ -310,33 +330,45 @@ struct extendible_hash_table {
           __builtin_amdgcn_s_waitcnt(0);  // value before key
           st(&b->keys[free_slot], (uint64_t)key);
           state = 1;
-        }
-        if (state == 0) {  // full: split, then try again
-          const uint32_t d = gallatin::utils::atomic_load(&b->depth);
-          if (!(d < max_bits && split(b, d))) state = 2;
+        } else if (d < max_bits && split(b, d, home)) {  // full: split, try again
+          d += 1;
+        } else {
+          state = 2;
         }
       }
-      unlock(b);
+      unlock(b, d);
     }
     if (state != 1) gallatin::utils::atomic_add(&failed, (uint64_t)1);
     return state == 1;
   }
 
-  // extendible_ht.cuh:1291-1375.
+  // extendible_ht.cuh:1291-1375.  The depth, home and keys are loaded
+  // together; a missing key is reported only if the depth is unchanged
+  // when looked at again after the keys (a split raises the depth before
+  // it clears moved keys), and a found key's value is taken only if the
+  // key is still in its slot after the value was read.
   __device__ bool query(Key key, Val& val) {
     const uint64_t h = hash(key);
     for (uint32_t tries = 0; tries < kTries; ++tries) {
       bucket* b = owner(h);
       if (!b) return false;
-      const uint32_t d0 = gallatin::utils::atomic_load(&b->depth);
-      if ((h & mask(d0)) != ld(&b->home)) continue;  // being split: look again
+      const uint32_t d0 = depth_of(ld(&b->ctl));
       int match, empty;
-      scan(b, (uint64_t)key, match, empty);
+      const uint64_t home = scan(b, (uint64_t)key, match, empty);
+      if ((h & mask(d0)) != home) continue;  // being split: look again
       if (match >= 0) {
-        val = (Val)ld(&b->vals[match]);
-        return true;
+        const uint64_t v = ld(&b->vals[match]);
+        __builtin_amdgcn_s_waitcnt(0);  // the value is read before the key is read again
+        // Between the two reads the key may have left the slot (moved by a
+        // split or removed) and another insert may have reused it, value
+        // first: the value belongs to the key only if the key is still there.
+        if (ld(&b->keys[match]) == (uint64_t)key) {
+          val = (Val)v;
+          return true;
+        }
+        continue;
       }
-      if (gallatin::utils::atomic_load(&b->depth) == d0) return false;
+      if (depth_of(ld(&b->ctl)) == d0) return false;
     }
     return false;
   }
@@ -348,20 +380,21 @@ struct extendible_hash_table {
     for (uint32_t tries = 0; state == 0 && tries < kTries; ++tries) {
       bucket* b = owner(h);
       if (!b) break;
-      if (!try_lock(b)) {
+      uint32_t d;
+      if (!try_lock(b, d)) {
         __builtin_amdgcn_s_sleep(2);
         continue;
       }
-      if (owns(b, h)) {
-        int match, empty;
-        scan(b, (uint64_t)key, match, empty);
+      int match, empty;
+      const uint64_t home = scan(b, (uint64_t)key, match, empty);
+      if ((h & mask(d)) == home) {
         state = 2;
         if (match >= 0) {
           st(&b->keys[match], (uint64_t)defaultKey);
           state = 1;
         }
       }
-      unlock(b);
+      unlock(b, d);
     }
     return state == 1;
   }

@@ -328,7 +328,8 @@ int gal_batch_plan_destroy(gal_batch_plan *plan);
  * for the whole device. */
 int gal_bench_atomic(uint32_t *words_dev, uint64_t span, uint64_t n_ops,
                      uint32_t lanes_per_address, uint32_t stride, void *stream);
-/* Copy of `bytes` (multiple of 16) with 16-byte loads and stores. */
+/* Copy of `bytes` (multiple of 16) with 16-byte loads and stores, each
+ * workgroup streaming contiguous 64 KiB tiles. */
 int gal_bench_copy(const void *src_dev, void *dst_dev, uint64_t bytes,
                    void *stream);
 
