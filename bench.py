@@ -90,8 +90,8 @@ def measured_traffic(kernel, args):
 
 def atomic_ceiling(G, torch):
     """Agent-scope atomicAdd(u32) with returned value: distinct consecutive
-    addresses, one address per 128-byte line, one address per wave; and a
-    16-byte-vector copy for the HBM ceiling."""
+    addresses, one address per 128-byte line, one address per wave, one
+    address for the whole device."""
     words = torch.zeros(1 << 26, dtype=torch.int32, device="cuda")
     out = {}
     for name, span, lanes, stride, nops in (
@@ -107,18 +107,37 @@ def atomic_ceiling(G, torch):
         e1.record()
         torch.cuda.synchronize()
         out[name] = nops / (e0.elapsed_time(e1) / 1e3)
+    del words
+    torch.cuda.empty_cache()
+    return out
+
+
+def hbm_ceiling(G, torch):
+    """Read+write rates (GB/s, both directions counted) of 2 GiB copies:
+    contiguous 64 KiB tiles per workgroup, and scattered pieces of 256 B and
+    2 KiB (pieces read and written at permuted positions, each wave walking
+    consecutive 16-byte vectors as the churn's fill and verify do). The
+    churn's payload bytes are spread evenly over allocation sizes 1..4096 B,
+    so the 2 KiB scattered copy is the rate its access pattern can reach."""
     src = torch.empty(1 << 31, dtype=torch.uint8, device="cuda")
     dst = torch.empty_like(src)
-    G.bench_copy(src, dst, src.numel())
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(5):
-        G.bench_copy(src, dst, src.numel())
-    e1.record()
-    torch.cuda.synchronize()
-    out["copy_gbs"] = 5 * 2 * src.numel() / (e0.elapsed_time(e1) / 1e3) / 1e9
-    del words, src, dst
+    out = {}
+
+    def rate(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return 5 * 2 * src.numel() / (e0.elapsed_time(e1) / 1e3) / 1e9
+
+    out["copy_gbs"] = rate(lambda: G.bench_copy(src, dst, src.numel()))
+    for piece in (256, 2048):
+        out[f"scatter_{piece}_gbs"] = rate(lambda: G.bench_scatter(src, dst, src.numel(), piece))
+    del src, dst
     torch.cuda.empty_cache()
     return out
 
@@ -368,6 +387,13 @@ def main():
             "achieved": atoms, "peak": ceil["per_line"], "unit": "atomics/s",
             "frac": atoms / ceil["per_line"], "per_pair": 2, "ceilings": ceil,
         }
+        hbm = hbm_ceiling(G, torch)
+        traffic = out["roofline"]["traffic"]
+        if traffic:
+            hbm["traffic_gbs"] = traffic / (head["ms_per_step"] / 1e3) / 1e9
+            hbm["traffic_frac_of_copy"] = hbm["traffic_gbs"] / hbm["copy_gbs"]
+            hbm["traffic_frac_of_scatter_2048"] = hbm["traffic_gbs"] / hbm["scatter_2048_gbs"]
+        out["hbm_ceilings"] = hbm
     out["plain_small"] = plain_small(args, G, torch)
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)

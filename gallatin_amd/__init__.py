@@ -43,6 +43,7 @@ CAPI_SYMBOLS = [
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
     "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
+    "gal_bench_scatter",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
     "gal_batch_plan_launch", "gal_batch_plan_destroy",
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
@@ -119,6 +120,7 @@ def lib():
         L.gal_run_churn_batched.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _p]
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
+        L.gal_bench_scatter.argtypes = [_p, _p, _u64, _u64, _p]
         L.gal_run_batches.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32, _u32, _p]
         L.gal_batch_plan_create.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32,
                                             _u32, ctypes.POINTER(_p)]
@@ -527,3 +529,8 @@ def bench_atomic(words, span, n_ops, lanes_per_address=1, stride=1):
 
 def bench_copy(src, dst, nbytes):
     _check(lib().gal_bench_copy(src.data_ptr(), dst.data_ptr(), nbytes, _stream()))
+
+
+def bench_scatter(src, dst, nbytes, piece_bytes):
+    _check(lib().gal_bench_scatter(src.data_ptr(), dst.data_ptr(), nbytes, piece_bytes,
+                                   _stream()))

@@ -325,6 +325,39 @@ __global__ void __launch_bounds__(kCopyThreads)
     dst[i] = src[i];
 }
 
+// Scattered-piece copy: the buffers are cut into 2^k pieces of
+// piece_vecs 16-byte vectors, and logical piece i is read from source piece
+// (i * ma) mod 2^k and written to destination piece (i * mb) mod 2^k (odd
+// multipliers, so both maps are bijections and every access is in bounds).
+// Each wave walks 64 * kCopyVecs consecutive logical vectors per step, the
+// same per-wave order as the churn's fill and verify, so this is the
+// bandwidth ceiling of that access pattern at a given allocation size.
+__global__ void __launch_bounds__(256)
+    k_bench_scatter(const u64x2* src_, u64x2* dst_, uint64_t pieces, uint32_t piece_vecs,
+                    uint64_t ma, uint64_t mb) {
+  const global_u64x2* src = (const global_u64x2*)src_;
+  global_u64x2* dst = (global_u64x2*)dst_;
+  const uint64_t mask = pieces - 1;
+  const uint64_t total = pieces * piece_vecs;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t base = wave * 64 * kCopyVecs; base < total; base += nwaves * 64 * kCopyVecs) {
+    u64x2 v[kCopyVecs];
+    uint64_t to[kCopyVecs];
+#pragma unroll
+    for (int k = 0; k < kCopyVecs; ++k) {
+      const uint64_t x = base + (uint64_t)k * 64 + lane;
+      const uint64_t piece = x / piece_vecs, off = x % piece_vecs;
+      to[k] = ((piece * mb) & mask) * piece_vecs + off;
+      if (x < total) v[k] = src[((piece * ma) & mask) * piece_vecs + off];
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyVecs; ++k)
+      if (base + (uint64_t)k * 64 + lane < total) dst[to[k]] = v[k];
+  }
+}
+
 static unsigned grid_for(uint64_t n, uint32_t threads) {
   return (unsigned)((n + threads - 1) / threads);
 }
@@ -434,6 +467,21 @@ int gal_bench_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
   k_bench_copy<<<(unsigned)cus * 8, kCopyThreads, 0, (hipStream_t)stream>>>(
       static_cast<const u64x2*>(src), static_cast<u64x2*>(dst), bytes / 16);
   return launch_status("bench_copy");
+}
+
+int gal_bench_scatter(const void* src, void* dst, uint64_t bytes, uint64_t piece_bytes,
+                      void* stream) {
+  const auto pow2 = [](uint64_t x) { return x && !(x & (x - 1)); };
+  if (!src || !dst || !pow2(bytes) || !pow2(piece_bytes) || piece_bytes < 16 ||
+      piece_bytes > bytes || (bytes / piece_bytes) < 2)
+    return fail(GAL_ERR_INVALID, "bad scatter arguments");
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  k_bench_scatter<<<(unsigned)cus * 8, 256, 0, (hipStream_t)stream>>>(
+      static_cast<const u64x2*>(src), static_cast<u64x2*>(dst), bytes / piece_bytes,
+      (uint32_t)(piece_bytes / 16), 0x9E3779B97F4A7C15ULL, 0xC2B2AE3D27D4EB4FULL);
+  return launch_status("bench_scatter");
 }
 
 }  // extern "C"

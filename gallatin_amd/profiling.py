@@ -10,6 +10,12 @@
         Each counter is scaled by what it reads for k_bench_copy moving B
         bytes (B read and B written) in the same run, so the result is in
         bytes actually moved, whatever unit the counter definition assumes.
+
+    python -m gallatin_amd.profiling counters <counter_collection.csv>...
+        [--dispatches KERNEL]
+        Markdown table of the mean of every counter per kernel over its
+        dispatches, merging several PMC runs of the same program; with
+        --dispatches, also each dispatch of KERNEL in launch order.
 """
 import argparse
 import collections
@@ -84,6 +90,44 @@ def traffic(fetch_csv, write_csv, copy_bytes, workload, match=None):
     return out
 
 
+def _counter_rows(paths):
+    """{kernel: {counter: [values in dispatch order]}} over several runs."""
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r.get("Dispatch_Id", 0)))
+        for r in rows:
+            out[kernel_key(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return out
+
+
+def counters_markdown(paths, dispatches=None):
+    data = _counter_rows(paths)
+    names = sorted({c for k in data.values() for c in k})
+    out = ["| kernel | dispatches | " + " | ".join(f"`{n}`" for n in names) + " |",
+           "|---|---|" + "---|" * len(names)]
+    for k in sorted(data):
+        if k.startswith("__amd") or k.startswith("at::"):
+            continue
+        cells = []
+        for n in names:
+            v = data[k].get(n)
+            cells.append(f"{sum(v) / len(v):,.0f}" if v else "")
+        n_disp = max(len(v) for v in data[k].values())
+        out.append(f"| `{k}` | {n_disp} | " + " | ".join(cells) + " |")
+    if dispatches and dispatches in data:
+        out.append("")
+        out.append(f"Dispatches of `{dispatches}` in launch order:")
+        out.append("")
+        out.append("| # | " + " | ".join(f"`{n}`" for n in names if n in data[dispatches]) + " |")
+        cols = [n for n in names if n in data[dispatches]]
+        out.append("|---|" + "---|" * len(cols))
+        for i in range(max(len(data[dispatches][n]) for n in cols)):
+            out.append(f"| {i} | " + " | ".join(
+                f"{data[dispatches][n][i]:,.0f}" if i < len(data[dispatches][n]) else ""
+                for n in cols) + " |")
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     p = argparse.ArgumentParser()
     sub = p.add_subparsers(dest="cmd", required=True)
@@ -99,9 +143,14 @@ def main(argv=None):
     t.add_argument("--match", default="",
                    help="key=int,... : the benchmark settings this holds for")
     t.add_argument("--out")
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--dispatches")
     a = p.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.title, a.command))
+    elif a.cmd == "counters":
+        sys.stdout.write(counters_markdown(a.csv, a.dispatches))
     else:
         match = dict((kv.split("=")[0], int(kv.split("=")[1]))
                      for kv in a.match.split(",") if kv)

@@ -332,6 +332,12 @@ int gal_bench_atomic(uint32_t *words_dev, uint64_t span, uint64_t n_ops,
  * workgroup streaming contiguous 64 KiB tiles. */
 int gal_bench_copy(const void *src_dev, void *dst_dev, uint64_t bytes,
                    void *stream);
+/* Copy of `bytes` cut into pieces of `piece_bytes` (both powers of two,
+ * piece >= 16 B), each piece read from and written to pseudo-randomly
+ * permuted positions: the bandwidth ceiling of payload traffic spread over
+ * scattered allocations of that size. */
+int gal_bench_scatter(const void *src_dev, void *dst_dev, uint64_t bytes,
+                      uint64_t piece_bytes, void *stream);
 
 /* Last error message of the calling thread (static storage). */
 const char *gal_last_error(void);
