@@ -51,6 +51,8 @@ CAPI_SYMBOLS = [
     "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
     "gal_serving_check", "gal_serving_table", "gal_serving_destroy",
     "gal_queue_create", "gal_queue_destroy", "gal_queue_enqueue", "gal_queue_dequeue",
+    "gal_chain_create", "gal_chain_destroy", "gal_chain_insert", "gal_chain_query",
+    "gal_chain_stats", "gal_chain_lengths", "gal_chain_print",
 ]
 
 
@@ -146,6 +148,14 @@ def lib():
         L.gal_queue_destroy.argtypes = [_p]
         L.gal_queue_enqueue.argtypes = [_p, _p, _u64, _p, ctypes.c_int, _p]
         L.gal_queue_dequeue.argtypes = [_p, _p, _p, _u64, ctypes.c_int, _p]
+        L.gal_chain_create.argtypes = [ctypes.c_int, _u64, _u64, _u64, ctypes.c_int,
+                                       ctypes.POINTER(_p)]
+        L.gal_chain_destroy.argtypes = [_p]
+        L.gal_chain_insert.argtypes = [_p, _p, _p, _u64, _p, _p]
+        L.gal_chain_query.argtypes = [_p, _p, _u64, _p, _p, _p]
+        L.gal_chain_stats.argtypes = [_p, _p, _p, _p, _p]
+        L.gal_chain_lengths.argtypes = [_p, _p, _p]
+        L.gal_chain_print.argtypes = [_p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -450,6 +460,72 @@ class Table:
     def destroy(self):
         if self.handle:
             _check(lib().gal_table_destroy(self.handle))
+            self.handle = _p()
+
+
+class ChainTable:
+    """chaining_table<Key, Val, size> (chaining_table.cuh:114-335) on the
+    global allocator. Kinds (key/value width, pairs per block): 0 (64-bit,
+    4), 1 (64-bit, 16), 2 (32-bit, 4), 3 (32-bit, 32), 4 (64-bit, 1).
+    Keys and values are int64 tensors for 64-bit kinds, int32 for 32-bit."""
+
+    KINDS = {0: (8, 4), 1: (8, 16), 2: (4, 4), 3: (4, 32), 4: (8, 1)}
+
+    def __init__(self, kind=0, nslots=1 << 20, default_key=0, seed=123456, prealloc=False):
+        if kind not in self.KINDS:
+            raise GallatinError(f"unknown chaining table kind {kind}")
+        self.kind = kind
+        self.key_bytes, self.block_pairs = self.KINDS[kind]
+        self.nblocks = nslots // self.block_pairs
+        self.handle = _p()
+        _check(lib().gal_chain_create(kind, nslots, default_key, seed, int(prealloc),
+                                      ctypes.byref(self.handle)))
+
+    def _dtype(self):
+        import torch
+        return torch.int64 if self.key_bytes == 8 else torch.int32
+
+    def _check_arg(self, t):
+        if t.dtype != self._dtype() or not t.is_cuda or not t.is_contiguous():
+            raise GallatinError(f"expected a contiguous {self._dtype()} tensor on the GPU")
+
+    def insert(self, keys, vals):
+        import torch
+        self._check_arg(keys)
+        self._check_arg(vals)
+        if keys.numel() != vals.numel():
+            raise GallatinError("keys and values differ in length")
+        ok = torch.zeros(keys.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_chain_insert(self.handle, keys.data_ptr(), vals.data_ptr(),
+                                      keys.numel(), ok.data_ptr(), _stream()))
+        return ok
+
+    def query(self, keys):
+        import torch
+        self._check_arg(keys)
+        vals = torch.zeros(keys.numel(), dtype=self._dtype(), device="cuda")
+        found = torch.zeros(keys.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_chain_query(self.handle, keys.data_ptr(), keys.numel(),
+                                     vals.data_ptr(), found.data_ptr(), _stream()))
+        return vals, found
+
+    def lengths(self):
+        import torch
+        out = torch.zeros(self.nblocks, dtype=torch.int32, device="cuda")
+        _check(lib().gal_chain_lengths(self.handle, out.data_ptr(), _stream()))
+        return out
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(lib().gal_chain_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("max_chain", "blocks", "nblocks", "failed"), (x.value for x in v)))
+
+    def print_chain_stats(self):
+        _check(lib().gal_chain_print(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().gal_chain_destroy(self.handle))
             self.handle = _p()
 
 

@@ -1,11 +1,15 @@
 // The library's global allocator instance and its bulk kernels
 // (global_allocator.cuh:41-97), and the allocator-backed data structures
-// that allocate through it (fixed_vector, extendible_hash_table).  This is
+// that allocate through it (fixed_vector, extendible_hash_table,
+// chaining_table).  This is
 // the only translation unit of the library that includes
 // global_allocator.hpp, which defines the device symbol global_gallatin.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "gallatin/allocators/global_allocator.hpp"
+#include "gallatin/data_structs/chaining_table.hpp"
 #include "gallatin/data_structs/extendible_ht.hpp"
 #include "gallatin/data_structs/fixed_vector.hpp"
 #include "gallatin_capi.h"
@@ -256,6 +260,160 @@ int gal_table_stats(gal_table* t, uint64_t* keys, uint64_t* buckets, uint64_t* l
   if (level) *level = hdr[1];
   if (failed) *failed = hdr[4];
   return GAL_OK;
+}
+
+}  // extern "C"
+
+// ---- chained hash tables ----
+//
+// Instantiations reachable through the C ABI, by kind: 64-bit keys and
+// values with 4 (the reference's default), 16 and 1 pairs per block, and
+// 32-bit keys and values with 4 and 32 pairs per block.
+
+using gallatin::data_structs::chaining_table;
+using chain_u64_4 = chaining_table<uint64_t, uint64_t, 4>;
+using chain_u64_16 = chaining_table<uint64_t, uint64_t, 16>;
+using chain_u32_4 = chaining_table<uint32_t, uint32_t, 4>;
+using chain_u32_32 = chaining_table<uint32_t, uint32_t, 32>;
+using chain_u64_1 = chaining_table<uint64_t, uint64_t, 1>;
+
+struct gal_chain {
+  int kind = 0;
+  void* dev = nullptr;
+  uint64_t nblocks = 0;
+};
+
+namespace {
+
+// Calls f with a null pointer of the kind's table type.
+template <typename F>
+int with_chain_kind(int kind, F&& f) {
+  switch (kind) {
+    case 0: return f(static_cast<chain_u64_4*>(nullptr));
+    case 1: return f(static_cast<chain_u64_16*>(nullptr));
+    case 2: return f(static_cast<chain_u32_4*>(nullptr));
+    case 3: return f(static_cast<chain_u32_32*>(nullptr));
+    case 4: return f(static_cast<chain_u64_1*>(nullptr));
+    default: return fail(GAL_ERR_INVALID, "unknown chaining table kind");
+  }
+}
+
+template <typename T>
+__global__ void k_chain_insert(T* t, const typename T::key_type* keys,
+                               const typename T::val_type* vals, uint64_t n, uint8_t* ok) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool r = t->insert(keys[i], vals[i]);
+  if (ok) ok[i] = r;
+}
+
+template <typename T>
+__global__ void k_chain_query(T* t, const typename T::key_type* keys, uint64_t n,
+                              typename T::val_type* vals, uint8_t* found) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typename T::val_type v = 0;
+  const bool r = t->query(keys[i], v);
+  if (vals) vals[i] = r ? v : 0;
+  if (found) found[i] = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gal_chain_create(int kind, uint64_t nslots, uint64_t default_key, uint64_t seed,
+                     int prealloc, gal_chain** out) {
+  if (!out) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  return with_chain_kind(kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    using K = typename T::key_type;
+    if (sizeof(K) == 4 && default_key > 0xFFFFFFFFULL)
+      return fail(GAL_ERR_INVALID, "default key does not fit a 32-bit key");
+    T* dev = prealloc ? T::generate_on_device_prealloc(nslots, (K)default_key, seed)
+                      : T::generate_on_device(nslots, (K)default_key, seed);
+    if (!dev) return fail(GAL_ERR_INVALID, "chaining table creation failed (nslots < block size?)");
+    *out = new gal_chain{kind, dev, nslots / T::block_pairs};
+    return GAL_OK;
+  });
+}
+
+int gal_chain_destroy(gal_chain* c) {
+  if (!c) return GAL_OK;
+  const int r = with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    T::free_on_device(static_cast<T*>(c->dev));
+    return GAL_OK;
+  });
+  delete c;
+  return r;
+}
+
+int gal_chain_insert(gal_chain* c, const void* keys, const void* vals, uint64_t n, uint8_t* ok,
+                     void* stream) {
+  if (!c || ((!keys || !vals) && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  return with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    k_chain_insert<T><<<blocks_for(n), 256, 0, (hipStream_t)stream>>>(
+        static_cast<T*>(c->dev), static_cast<const typename T::key_type*>(keys),
+        static_cast<const typename T::val_type*>(vals), n, ok);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GAL_OK : hip_fail(e, "chain insert");
+  });
+}
+
+int gal_chain_query(gal_chain* c, const void* keys, uint64_t n, void* vals, uint8_t* found,
+                    void* stream) {
+  if (!c || (!keys && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  return with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    k_chain_query<T><<<blocks_for(n), 256, 0, (hipStream_t)stream>>>(
+        static_cast<T*>(c->dev), static_cast<const typename T::key_type*>(keys), n,
+        static_cast<typename T::val_type*>(vals), found);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GAL_OK : hip_fail(e, "chain query");
+  });
+}
+
+int gal_chain_stats(gal_chain* c, uint64_t* max_len, uint64_t* total_blocks, uint64_t* nblocks,
+                    uint64_t* failed) {
+  if (!c) return fail(GAL_ERR_INVALID, "null argument");
+  if (hipDeviceSynchronize() != hipSuccess) return fail(GAL_ERR_HIP, "sync");
+  return with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    T* dev = static_cast<T*>(c->dev);
+    T host;
+    if (hipMemcpy(&host, dev, sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(GAL_ERR_HIP, "stats copy");
+    dev->chain_stats(max_len, total_blocks);
+    if (nblocks) *nblocks = host.nblocks;
+    if (failed) *failed = host.failed;
+    return hipDeviceSynchronize() == hipSuccess ? GAL_OK : fail(GAL_ERR_HIP, "stats");
+  });
+}
+
+int gal_chain_lengths(gal_chain* c, uint32_t* out, void* stream) {
+  if (!c || !out) return fail(GAL_ERR_INVALID, "null argument");
+  return with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    gallatin::data_structs::chain_lengths_kernel<T>
+        <<<blocks_for(c->nblocks), 256, 0, (hipStream_t)stream>>>(static_cast<T*>(c->dev), out,
+                                                                  c->nblocks);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GAL_OK : hip_fail(e, "chain lengths");
+  });
+}
+
+int gal_chain_print(gal_chain* c) {
+  if (!c) return fail(GAL_ERR_INVALID, "null argument");
+  return with_chain_kind(c->kind, [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    static_cast<T*>(c->dev)->print_chain_stats();
+    return GAL_OK;
+  });
 }
 
 }  // extern "C"

@@ -241,6 +241,33 @@ int gal_table_remove(gal_table *t, const uint64_t *keys_dev, uint64_t n,
 int gal_table_stats(gal_table *t, uint64_t *keys, uint64_t *buckets,
                     uint64_t *level, uint64_t *failed);
 
+/* gal_chain: chaining_table<Key, Val, size> (data_structs/chaining_table.cuh
+ * :114-335, header include/gallatin/data_structs/chaining_table.hpp).
+ * kind 0: 64-bit keys and values, 4 pairs per block (the reference's
+ * default); 1: 64-bit, 16 pairs; 2: 32-bit, 4 pairs; 3: 32-bit, 32 pairs;
+ * 4: 64-bit, 1 pair.  Keys and values are arrays of the kind's width.
+ * create: nslots / size chains, key default_key marks an empty slot
+ * (generate_on_device, chaining_table.cuh:132-156); prealloc != 0 gives
+ * every chain its first block (generate_on_device_prealloc, :158-187).
+ * insert (:240-282) stores a pair, duplicates included; ok[i] = 0 for
+ * default_key or when no block could be allocated.  query (:284-309)
+ * returns the first ready pair of the key in chain order.  stats: longest
+ * chain and total blocks (print_chain_stats, :295-322), chains, refused
+ * inserts; synchronous.  lengths: blocks of each chain, into nblocks
+ * uint32 on the device.  destroy also returns the chain blocks. */
+typedef struct gal_chain gal_chain;
+int gal_chain_create(int kind, uint64_t nslots, uint64_t default_key, uint64_t seed,
+                     int prealloc, gal_chain **out);
+int gal_chain_destroy(gal_chain *c);
+int gal_chain_insert(gal_chain *c, const void *keys_dev, const void *vals_dev, uint64_t n,
+                     uint8_t *ok_dev, void *stream);
+int gal_chain_query(gal_chain *c, const void *keys_dev, uint64_t n, void *vals_dev,
+                    uint8_t *found_dev, void *stream);
+int gal_chain_stats(gal_chain *c, uint64_t *max_len, uint64_t *total_blocks,
+                    uint64_t *nblocks, uint64_t *failed);
+int gal_chain_lengths(gal_chain *c, uint32_t *out_dev, void *stream);
+int gal_chain_print(gal_chain *c);
+
 /* gal_queue: queue<uint64_t, Gallatin<...>> (data_structs/queue.cuh:59-192,
  * header include/gallatin/data_structs/queue.hpp) on allocator `a`, or with
  * ring != 0 the ring of 8 such queues (circular_CAS_queue, queue.cuh:196).

@@ -50,6 +50,8 @@ def lib():
         L.orc_cfcll.argtypes = [_u64]
         L.orc_murmur64a.restype = _u64
         L.orc_murmur64a.argtypes = [_p, ctypes.c_int, _u64]
+        L.orc_chain_of.restype = None
+        L.orc_chain_of.argtypes = [_p, ctypes.c_int, _u64, _u64, _u64, _p]
         L.orc_veb_create.restype = _p
         L.orc_veb_create.argtypes = [_u64, ctypes.c_int]
         L.orc_veb_destroy.argtypes = [_p]
@@ -101,6 +103,15 @@ def lib():
 def murmur64a(data: bytes, seed: int) -> int:
     buf = ctypes.create_string_buffer(data, len(data))
     return lib().orc_murmur64a(buf, len(data), seed)
+
+
+def chain_of(keys, seed: int, nchains: int):
+    """Chain index of each key of a numpy array of uint32 / uint64 keys."""
+    import numpy as np
+    k = np.ascontiguousarray(keys)
+    out = np.zeros(len(k), dtype=np.uint64)
+    lib().orc_chain_of(k.ctypes.data, k.dtype.itemsize, len(k), seed, nchains, out.ctypes.data)
+    return out
 
 
 def clog2(x: int) -> int:

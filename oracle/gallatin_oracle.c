@@ -38,6 +38,16 @@ int orc_clog2(uint64_t x) {
   return 63 - __builtin_clzll(x) + (__builtin_popcountll(x) != 1);
 }
 
+/* chaining_table.cuh:244 and :290: the chain of a key is MurmurHash64A over
+ * the key's bytes (key_bytes = sizeof(Key), little-endian), modulo the
+ * number of chains.  out[i] for keys[i], n keys packed back to back. */
+void orc_chain_of(const void *keys, int key_bytes, uint64_t n, uint64_t seed,
+                  uint64_t nchains, uint64_t *out) {
+  const unsigned char *p = (const unsigned char *)keys;
+  for (uint64_t i = 0; i < n; i++)
+    out[i] = orc_murmur64a(p + i * (uint64_t)key_bytes, key_bytes, seed) % nchains;
+}
+
 /* alloc_utils.cuh:440-453 __cfcll: number of trailing one bits. */
 int orc_cfcll(uint64_t bits) {
   int p = __builtin_popcountll(bits);

@@ -61,6 +61,8 @@ extern "C" {
 int orc_clog2(uint64_t x);
 int orc_cfcll(uint64_t bits);
 uint64_t orc_murmur64a(const void *key, int len, uint64_t seed);
+void orc_chain_of(const void *keys, int key_bytes, uint64_t n, uint64_t seed,
+                  uint64_t nchains, uint64_t *out);
 
 /* ------------------------------------------------------------- vEB tree */
 typedef struct orc_veb {

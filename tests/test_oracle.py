@@ -20,6 +20,17 @@ def test_murmur64a_smhasher_verification():
     assert final == int(G["murmur64a_smhasher_verification"], 16)
 
 
+@pytest.mark.parametrize("dt,fmt", [(np.uint32, "<I"), (np.uint64, "<Q")])
+def test_chain_of_is_murmur_of_key_bytes(dt, fmt):
+    """chain_of (the chaining table's bucket choice) hashes each key's
+    little-endian bytes with the KAT-pinned murmur64a, modulo the chains."""
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, np.iinfo(dt).max, 500, dtype=dt)
+    got = oracle.chain_of(keys, 123456, 997)
+    exp = [oracle.murmur64a(struct.pack(fmt, int(k)), 123456) % 997 for k in keys]
+    assert np.array_equal(got, np.array(exp, dtype=np.uint64))
+
+
 def test_clog2_closed_form():
     for x in list(range(1, 5000)) + [2 ** k + d for k in range(12, 63) for d in (-1, 0, 1)]:
         assert oracle.clog2(x) == H.clog2(x), x
