@@ -53,6 +53,8 @@ CAPI_SYMBOLS = [
     "gal_queue_create", "gal_queue_destroy", "gal_queue_enqueue", "gal_queue_dequeue",
     "gal_chain_create", "gal_chain_destroy", "gal_chain_insert", "gal_chain_query",
     "gal_chain_stats", "gal_chain_lengths", "gal_chain_print",
+    "gal_bqueue_create", "gal_bqueue_destroy", "gal_bqueue_enqueue", "gal_bqueue_dequeue",
+    "gal_bqueue_reclaim", "gal_bqueue_stats",
 ]
 
 
@@ -156,6 +158,12 @@ def lib():
         L.gal_chain_stats.argtypes = [_p, _p, _p, _p, _p]
         L.gal_chain_lengths.argtypes = [_p, _p, _p]
         L.gal_chain_print.argtypes = [_p]
+        L.gal_bqueue_create.argtypes = [ctypes.c_int, ctypes.POINTER(_p)]
+        L.gal_bqueue_destroy.argtypes = [_p]
+        L.gal_bqueue_enqueue.argtypes = [_p, _p, _u64, _p, ctypes.c_int, _p]
+        L.gal_bqueue_dequeue.argtypes = [_p, _p, _p, _u64, ctypes.c_int, _p]
+        L.gal_bqueue_reclaim.argtypes = [_p, _p]
+        L.gal_bqueue_stats.argtypes = [_p, _p, _p, _p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -526,6 +534,50 @@ class ChainTable:
     def destroy(self):
         if self.handle:
             _check(lib().gal_chain_destroy(self.handle))
+            self.handle = _p()
+
+
+class BlockQueue:
+    """block_queue<uint64_t, items_per_block> (block_queue.cuh:240-463) on the
+    global allocator; kind 0: 256 items per node, kind 1: 16."""
+
+    ITEMS = {0: 256, 1: 16}
+
+    def __init__(self, kind=0):
+        if kind not in self.ITEMS:
+            raise GallatinError(f"unknown block queue kind {kind}")
+        self.items_per_block = self.ITEMS[kind]
+        self.handle = _p()
+        _check(lib().gal_bqueue_create(kind, ctypes.byref(self.handle)))
+
+    def enqueue(self, items, serial=False):
+        import torch
+        ok = torch.zeros(items.numel(), dtype=torch.uint8, device="cuda")
+        _check(lib().gal_bqueue_enqueue(self.handle, items.data_ptr(), items.numel(),
+                                        ok.data_ptr(), int(serial), _stream()))
+        return ok
+
+    def dequeue(self, n, serial=False):
+        import torch
+        out = torch.zeros(n, dtype=torch.int64, device="cuda")
+        ok = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        _check(lib().gal_bqueue_dequeue(self.handle, out.data_ptr(), ok.data_ptr(), n,
+                                        int(serial), _stream()))
+        return out, ok
+
+    def reclaim(self):
+        v = ctypes.c_uint64()
+        _check(lib().gal_bqueue_reclaim(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(3)]
+        _check(lib().gal_bqueue_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("nodes", "retired", "failed"), (x.value for x in v)))
+
+    def destroy(self):
+        if self.handle:
+            _check(lib().gal_bqueue_destroy(self.handle))
             self.handle = _p()
 
 

@@ -1,7 +1,7 @@
 // The library's global allocator instance and its bulk kernels
 // (global_allocator.cuh:41-97), and the allocator-backed data structures
 // that allocate through it (fixed_vector, extendible_hash_table,
-// chaining_table).  This is
+// chaining_table, block_queue).  This is
 // the only translation unit of the library that includes
 // global_allocator.hpp, which defines the device symbol global_gallatin.
 #include <hip/hip_runtime.h>
@@ -9,6 +9,7 @@
 #include <type_traits>
 
 #include "gallatin/allocators/global_allocator.hpp"
+#include "gallatin/data_structs/block_queue.hpp"
 #include "gallatin/data_structs/chaining_table.hpp"
 #include "gallatin/data_structs/extendible_ht.hpp"
 #include "gallatin/data_structs/fixed_vector.hpp"
@@ -412,6 +413,166 @@ int gal_chain_print(gal_chain* c) {
   return with_chain_kind(c->kind, [&](auto* tag) -> int {
     using T = std::remove_pointer_t<decltype(tag)>;
     static_cast<T*>(c->dev)->print_chain_stats();
+    return GAL_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---- block queues ----
+//
+// block_queue<uint64_t, 256> (the reference test's node size) and
+// block_queue<uint64_t, 16> (small nodes: many node changes), by kind.
+
+using gallatin::data_structs::block_queue;
+using bqueue_256 = block_queue<uint64_t, 256>;
+using bqueue_16 = block_queue<uint64_t, 16>;
+
+struct gal_bqueue {
+  int kind = 0;
+  void* dev = nullptr;
+};
+
+namespace {
+
+template <typename F>
+int with_bqueue_kind(int kind, F&& f) {
+  switch (kind) {
+    case 0: return f(static_cast<bqueue_256*>(nullptr));
+    case 1: return f(static_cast<bqueue_16*>(nullptr));
+    default: return fail(GAL_ERR_INVALID, "unknown block queue kind");
+  }
+}
+
+template <typename Q>
+__global__ void k_bq_enqueue(Q* q, const uint64_t* items, uint64_t n, uint8_t* ok, int serial) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (serial) {  // one thread, in order
+    if (i == 0)
+      for (uint64_t k = 0; k < n; ++k) {
+        const bool r = q->enqueue(items[k]);
+        if (ok) ok[k] = r;
+      }
+    return;
+  }
+  if (i >= n) return;
+  const bool r = q->enqueue(items[i]);
+  if (ok) ok[i] = r;
+}
+
+template <typename Q>
+__global__ void k_bq_dequeue(Q* q, uint64_t* out, uint8_t* ok, uint64_t n, int serial) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (serial) {
+    if (i == 0)
+      for (uint64_t k = 0; k < n; ++k) {
+        uint64_t v = 0;
+        const bool r = q->dequeue(v);
+        out[k] = r ? v : 0;
+        if (ok) ok[k] = r;
+      }
+    return;
+  }
+  if (i >= n) return;
+  uint64_t v = 0;
+  const bool r = q->dequeue(v);
+  out[i] = r ? v : 0;
+  if (ok) ok[i] = r;
+}
+
+// Nodes in the list, nodes on the retired list, refused enqueues.
+template <typename Q>
+__global__ void k_bq_stats(Q* q, uint64_t* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t live = 0, retired = 0;
+  for (uint64_t a = q->head; a > Q::kAppending && live < Q::kHops; a = Q::as_node(a)->next)
+    ++live;
+  for (uint64_t a = q->retired; a && retired < Q::kHops; a = Q::as_node(a)->retired) ++retired;
+  out[0] = live;
+  out[1] = retired;
+  out[2] = q->failed;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gal_bqueue_create(int kind, gal_bqueue** out) {
+  if (!out) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  return with_bqueue_kind(kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    Q* dev = Q::generate_on_device();
+    if (!dev) return fail(GAL_ERR_HIP, "block queue creation failed");
+    *out = new gal_bqueue{kind, dev};
+    return GAL_OK;
+  });
+}
+
+int gal_bqueue_destroy(gal_bqueue* q) {
+  if (!q) return GAL_OK;
+  const int r = with_bqueue_kind(q->kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    Q::free_on_device(static_cast<Q*>(q->dev));
+    return GAL_OK;
+  });
+  delete q;
+  return r;
+}
+
+int gal_bqueue_enqueue(gal_bqueue* q, const uint64_t* items, uint64_t n, uint8_t* ok,
+                       int serial, void* stream) {
+  if (!q || (!items && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  return with_bqueue_kind(q->kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    k_bq_enqueue<Q><<<serial ? 1u : blocks_for(n), 256, 0, (hipStream_t)stream>>>(
+        static_cast<Q*>(q->dev), items, n, ok, serial);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GAL_OK : hip_fail(e, "block queue enqueue");
+  });
+}
+
+int gal_bqueue_dequeue(gal_bqueue* q, uint64_t* out, uint8_t* ok, uint64_t n, int serial,
+                       void* stream) {
+  if (!q || (!out && n)) return fail(GAL_ERR_INVALID, "null argument");
+  if (n == 0) return GAL_OK;
+  return with_bqueue_kind(q->kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    k_bq_dequeue<Q><<<serial ? 1u : blocks_for(n), 256, 0, (hipStream_t)stream>>>(
+        static_cast<Q*>(q->dev), out, ok, n, serial);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GAL_OK : hip_fail(e, "block queue dequeue");
+  });
+}
+
+int gal_bqueue_reclaim(gal_bqueue* q, uint64_t* freed) {
+  if (!q) return fail(GAL_ERR_INVALID, "null argument");
+  if (hipDeviceSynchronize() != hipSuccess) return fail(GAL_ERR_HIP, "sync");
+  return with_bqueue_kind(q->kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    const uint64_t n = Q::reclaim_on_device(static_cast<Q*>(q->dev), false);
+    if (freed) *freed = n;
+    return hipDeviceSynchronize() == hipSuccess ? GAL_OK : fail(GAL_ERR_HIP, "reclaim");
+  });
+}
+
+int gal_bqueue_stats(gal_bqueue* q, uint64_t* nodes, uint64_t* retired, uint64_t* failed) {
+  if (!q) return fail(GAL_ERR_INVALID, "null argument");
+  if (hipDeviceSynchronize() != hipSuccess) return fail(GAL_ERR_HIP, "sync");
+  return with_bqueue_kind(q->kind, [&](auto* tag) -> int {
+    using Q = std::remove_pointer_t<decltype(tag)>;
+    uint64_t* d = nullptr;
+    uint64_t h[3] = {0, 0, 0};
+    hipError_t e = hipMalloc(&d, sizeof(h));
+    if (e != hipSuccess) return hip_fail(e, "stats buffer");
+    k_bq_stats<Q><<<1, 64>>>(static_cast<Q*>(q->dev), d);
+    e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "stats copy");
+    if (nodes) *nodes = h[0];
+    if (retired) *retired = h[1];
+    if (failed) *failed = h[2];
     return GAL_OK;
   });
 }

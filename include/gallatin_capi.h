@@ -268,6 +268,26 @@ int gal_chain_stats(gal_chain *c, uint64_t *max_len, uint64_t *total_blocks,
 int gal_chain_lengths(gal_chain *c, uint32_t *out_dev, void *stream);
 int gal_chain_print(gal_chain *c);
 
+/* gal_bqueue: block_queue<uint64_t, items_per_block> (data_structs/
+ * block_queue.cuh:240-463, header include/gallatin/data_structs/
+ * block_queue.hpp) on the global allocator; kind 0: 256 items per node (the
+ * reference test's size), kind 1: 16.  enqueue (block_queue.cuh:331-405):
+ * ok[i] = 0 when no node could be allocated.  dequeue has no working
+ * reference counterpart (its code is commented out, :409-457): ok[i] = 0
+ * when the queue was empty.  serial != 0: one thread does all n operations
+ * in order.  reclaim returns the nodes that dequeues emptied (synchronous;
+ * no kernel may use the queue).  stats: nodes in the list, retired nodes,
+ * refused enqueues.  destroy returns every node. */
+typedef struct gal_bqueue gal_bqueue;
+int gal_bqueue_create(int kind, gal_bqueue **out);
+int gal_bqueue_destroy(gal_bqueue *q);
+int gal_bqueue_enqueue(gal_bqueue *q, const uint64_t *items_dev, uint64_t n,
+                       uint8_t *ok_dev, int serial, void *stream);
+int gal_bqueue_dequeue(gal_bqueue *q, uint64_t *out_dev, uint8_t *ok_dev, uint64_t n,
+                       int serial, void *stream);
+int gal_bqueue_reclaim(gal_bqueue *q, uint64_t *freed);
+int gal_bqueue_stats(gal_bqueue *q, uint64_t *nodes, uint64_t *retired, uint64_t *failed);
+
 /* gal_queue: queue<uint64_t, Gallatin<...>> (data_structs/queue.cuh:59-192,
  * header include/gallatin/data_structs/queue.hpp) on allocator `a`, or with
  * ring != 0 the ring of 8 such queues (circular_CAS_queue, queue.cuh:196).
