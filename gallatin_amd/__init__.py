@@ -25,6 +25,8 @@ FAIL = (1 << 64) - 1
 
 GAL_FLAG_DETERMINISTIC = 1
 GAL_FLAG_NO_PAYLOAD_FENCE = 2
+GAL_FLAG_POOL_HOST = 4
+GAL_FLAG_POOL_MANAGED = 8
 
 CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
 CTR_TIMEOUTS, CTR_FREES_SKIPPED = 5, 6
@@ -55,6 +57,11 @@ CAPI_SYMBOLS = [
     "gal_chain_stats", "gal_chain_lengths", "gal_chain_print",
     "gal_bqueue_create", "gal_bqueue_destroy", "gal_bqueue_enqueue", "gal_bqueue_dequeue",
     "gal_bqueue_reclaim", "gal_bqueue_stats",
+    "gal_init_global_host", "gal_free_global_host", "gal_print_global_stats_host",
+    "gal_global_host_handle", "gal_global_malloc_fused_bulk", "gal_global_free_fused_bulk",
+    "gal_global_malloc_combined_bulk", "gal_global_free_combined_bulk",
+    "gal_global_malloc_poison_bulk", "gal_global_check_poison_bulk",
+    "gal_global_free_poison_bulk",
 ]
 
 
@@ -164,6 +171,15 @@ def lib():
         L.gal_bqueue_dequeue.argtypes = [_p, _p, _p, _u64, ctypes.c_int, _p]
         L.gal_bqueue_reclaim.argtypes = [_p, _p]
         L.gal_bqueue_stats.argtypes = [_p, _p, _p, _p]
+        L.gal_init_global_host.argtypes = [_u64, _u64, ctypes.c_int]
+        L.gal_global_host_handle.restype = _p
+        L.gal_global_malloc_fused_bulk.argtypes = [_p, _u64, _p, _p]
+        L.gal_global_free_fused_bulk.argtypes = [_p, _u64, _p]
+        L.gal_global_malloc_combined_bulk.argtypes = [_p, _u64, ctypes.c_int, _p, _p]
+        L.gal_global_free_combined_bulk.argtypes = [_p, _u64, ctypes.c_int, _p]
+        L.gal_global_malloc_poison_bulk.argtypes = [_p, _u64, _p, _p]
+        L.gal_global_check_poison_bulk.argtypes = [_p, _u64, _p, _p]
+        L.gal_global_free_poison_bulk.argtypes = [_p, _u64, _p]
         L.gal_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -222,6 +238,18 @@ class Gallatin:
         self.flags = flags
         self._info_static()
         return self
+
+    def generate_on_device_host(self, max_bytes, seed, print_info=True, running_calloc=False,
+                                **kw):
+        """The pool in pinned, mapped host memory (gallatin.cuh:427-533)."""
+        kw["flags"] = kw.get("flags", 0) | GAL_FLAG_POOL_HOST
+        return self.generate_on_device(max_bytes, seed, print_info, running_calloc, **kw)
+
+    def generate_on_device_managed(self, max_bytes, seed, print_info=True,
+                                   running_calloc=False, **kw):
+        """The pool in managed memory (gallatin.cuh:536-630)."""
+        kw["flags"] = kw.get("flags", 0) | GAL_FLAG_POOL_MANAGED
+        return self.generate_on_device(max_bytes, seed, print_info, running_calloc, **kw)
 
     def _info_static(self):
         i = self.info()
@@ -648,6 +676,99 @@ def global_malloc_bulk(sizes, block_threads=256):
 def global_free_bulk(ptrs, block_threads=256):
     _check(lib().gal_global_free_bulk(ptrs.data_ptr(), ptrs.numel(),
                                       block_threads, _stream()))
+
+
+# ---- the host-pool instance, combined, fused, poison (global_allocator.cuh:100-357) ----
+
+def init_global_allocator_host(num_bytes, seed, print_info=True, running_calloc=False):
+    if running_calloc:
+        raise GallatinError("calloc mode is not supported")
+    _check(lib().gal_init_global_host(num_bytes, seed, int(bool(print_info))))
+
+
+def free_global_allocator_host():
+    _check(lib().gal_free_global_host())
+
+
+def print_global_stats_host():
+    _check(lib().gal_print_global_stats_host())
+
+
+def global_host_allocator():
+    """Non-owning view of the host-pool global instance."""
+    h = lib().gal_global_host_handle()
+    if not h:
+        raise GallatinError("no host allocator")
+    g = Gallatin()
+    g.handle = h
+    g.flags = GAL_FLAG_POOL_HOST
+    g._info_static()
+    return g
+
+
+def init_global_allocator_combined(num_bytes, host_bytes, seed, print_info=True,
+                                   running_calloc=False):
+    init_global_allocator(num_bytes, seed, print_info, running_calloc)
+    init_global_allocator_host(host_bytes, seed, print_info, running_calloc)
+
+
+def free_global_allocator_combined():
+    free_global_allocator()
+    free_global_allocator_host()
+
+
+def print_global_stats_combined():
+    print("Device Allocator:", flush=True)
+    print_global_stats()
+    print("Host Allocator:", flush=True)
+    print_global_stats_host()
+
+
+def _ptrs_out(n):
+    import torch
+    return torch.zeros(n, dtype=torch.int64, device="cuda")
+
+
+def global_malloc_fused_bulk(sizes):
+    out = _ptrs_out(sizes.numel())
+    _check(lib().gal_global_malloc_fused_bulk(sizes.data_ptr(), sizes.numel(), out.data_ptr(),
+                                              _stream()))
+    return out
+
+
+def global_free_fused_bulk(ptrs):
+    _check(lib().gal_global_free_fused_bulk(ptrs.data_ptr(), ptrs.numel(), _stream()))
+
+
+def global_malloc_combined_bulk(sizes, on_host=False):
+    out = _ptrs_out(sizes.numel())
+    _check(lib().gal_global_malloc_combined_bulk(sizes.data_ptr(), sizes.numel(),
+                                                 int(bool(on_host)), out.data_ptr(), _stream()))
+    return out
+
+
+def global_free_combined_bulk(ptrs, on_host=False):
+    _check(lib().gal_global_free_combined_bulk(ptrs.data_ptr(), ptrs.numel(),
+                                               int(bool(on_host)), _stream()))
+
+
+def global_malloc_poison_bulk(sizes):
+    out = _ptrs_out(sizes.numel())
+    _check(lib().gal_global_malloc_poison_bulk(sizes.data_ptr(), sizes.numel(), out.data_ptr(),
+                                               _stream()))
+    return out
+
+
+def global_check_poison_bulk(ptrs):
+    import torch
+    ok = torch.zeros(ptrs.numel(), dtype=torch.uint8, device="cuda")
+    _check(lib().gal_global_check_poison_bulk(ptrs.data_ptr(), ptrs.numel(), ok.data_ptr(),
+                                              _stream()))
+    return ok
+
+
+def global_free_poison_bulk(ptrs):
+    _check(lib().gal_global_free_poison_bulk(ptrs.data_ptr(), ptrs.numel(), _stream()))
 
 
 def bench_atomic(words, span, n_ops, lanes_per_address=1, stride=1):

@@ -198,9 +198,45 @@ void gal_config_default(gal_config* cfg, uint64_t max_bytes, uint64_t seed) {
   cfg->print_info = 0;
 }
 
+// The pool in device memory, pinned mapped host memory or managed memory.
+static hipError_t alloc_pool(gal_allocator* a) {
+  if (a->pool_kind != 0) {
+    int dev = 0, ok = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+      e = hipDeviceGetAttribute(&ok,
+                                a->pool_kind == 1 ? hipDeviceAttributeCanMapHostMemory
+                                                  : hipDeviceAttributeManagedMemory,
+                                dev);
+    if (e != hipSuccess) return e;
+    if (!ok) return hipErrorNotSupported;
+  }
+  if (a->pool_kind == 1) {
+    void* host = nullptr;
+    hipError_t e = hipHostMalloc(&host, a->pool_bytes, hipHostMallocMapped);
+    if (e != hipSuccess) return e;
+    void* dev = nullptr;
+    e = hipHostGetDevicePointer(&dev, host, 0);
+    if (e != hipSuccess || dev != host) {  // one address for host and device
+      (void)hipHostFree(host);
+      return e != hipSuccess ? e : hipErrorInvalidValue;
+    }
+    a->pool = host;
+    return hipSuccess;
+  }
+  if (a->pool_kind == 2) return hipMallocManaged(&a->pool, a->pool_bytes);
+  return hipMalloc(&a->pool, a->pool_bytes);
+}
+
+static hipError_t free_pool(gal_allocator* a) {
+  return a->pool_kind == 1 ? hipHostFree(a->pool) : hipFree(a->pool);
+}
+
 int gal_create(const gal_config* cfg, gal_allocator** out) {
   if (!cfg || !out) return fail(GAL_ERR_INVALID, "null argument");
   *out = nullptr;
+  if ((cfg->flags & GAL_FLAG_POOL_HOST) && (cfg->flags & GAL_FLAG_POOL_MANAGED))
+    return fail(GAL_ERR_INVALID, "a pool is either pinned host or managed memory, not both");
   const uint64_t S = cfg->bytes_per_segment, m = cfg->smallest, M = cfg->biggest;
   if (!is_pow2(S) || !is_pow2(m) || !is_pow2(M) || m > M)
     return fail(GAL_ERR_INVALID, "segment, smallest and biggest must be powers of two, smallest <= biggest");
@@ -298,11 +334,12 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
     return hip_fail(e, "hipMalloc(metadata)");
   }
   a->pool_bytes = N * S;
-  e = hipMalloc(&a->pool, a->pool_bytes);
+  a->pool_kind = (cfg->flags & GAL_FLAG_POOL_HOST) ? 1 : (cfg->flags & GAL_FLAG_POOL_MANAGED) ? 2 : 0;
+  e = alloc_pool(a);
   if (e != hipSuccess) {
     (void)hipFree(a->meta);
     delete a;
-    return hip_fail(e, "hipMalloc(pool)");
+    return hip_fail(e, "pool allocation");
   }
   char* base = static_cast<char*>(a->meta);
   h.memory = static_cast<char*>(a->pool);
@@ -320,9 +357,13 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
     h.sub_trees[t].words = tree_base + reinterpret_cast<uint64_t>(h.sub_trees[t].words);
   a->dev = reinterpret_cast<gal_state*>(base + o_state);
 
-  // the pool starts zeroed (memory_table.cuh:518)
+  // the pool starts zeroed (memory_table.cuh:518; a host pool is zeroed by
+  // the host, memory_table.cuh:537)
   int rc = GAL_OK;
-  if ((e = hipMemset(a->pool, 0, a->pool_bytes)) != hipSuccess) rc = hip_fail(e, "hipMemset(pool)");
+  if (a->pool_kind == 1)
+    std::memset(a->pool, 0, a->pool_bytes);
+  else if ((e = hipMemset(a->pool, 0, a->pool_bytes)) != hipSuccess)
+    rc = hip_fail(e, "hipMemset(pool)");
   if (rc == GAL_OK && (e = hipMemset(a->meta, 0, a->meta_bytes)) != hipSuccess)
     rc = hip_fail(e, "hipMemset(metadata)");
   if (rc == GAL_OK && (e = hipMemset(h.slots, 0xFF, (size_t)total * 4)) != hipSuccess)
@@ -347,7 +388,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
       rc = fail(GAL_ERR_BOOT, "pool too small for the pinned boot blocks");
   }
   if (rc != GAL_OK) {
-    (void)hipFree(a->pool);
+    (void)free_pool(a);
     (void)hipFree(a->meta);
     delete a;
     return rc;
@@ -357,9 +398,11 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
     g_registry[a->dev] = a;
   }
   if (cfg->print_info)
-    printf("Booted Gallatin with %u trees in range %llu-%llu and %f GB of memory %llu segments\n",
+    printf("Booted Gallatin with %u trees in range %llu-%llu and %f GB of %smemory %llu segments\n",
            T, (unsigned long long)m, (unsigned long long)M,
-           1.0 * (double)cfg->max_bytes / 1024 / 1024 / 1024, (unsigned long long)N);
+           1.0 * (double)cfg->max_bytes / 1024 / 1024 / 1024,
+           a->pool_kind == 1 ? "pinned host " : a->pool_kind == 2 ? "managed " : "",
+           (unsigned long long)N);
   *out = a;
   return GAL_OK;
 }
@@ -371,7 +414,7 @@ int gal_destroy(gal_allocator* a) {
     g_registry.erase(a->dev);
   }
   hipError_t e1 = hipDeviceSynchronize();
-  hipError_t e2 = hipFree(a->pool);
+  hipError_t e2 = free_pool(a);
   hipError_t e3 = hipFree(a->meta);
   delete a;
   if (e1 != hipSuccess) return hip_fail(e1, "hipDeviceSynchronize");

@@ -31,6 +31,49 @@ __global__ void k_global_free_bulk(void* const* ptrs, uint64_t n) {
   if (ptrs[i]) global_free(ptrs[i]);
 }
 
+__global__ void k_global_malloc_fused_bulk(const uint64_t* sizes, uint64_t n, void** out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = global_malloc_fused(sizes[i]);
+}
+
+__global__ void k_global_free_fused_bulk(void* const* ptrs, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && ptrs[i]) global_free_fused(ptrs[i]);
+}
+
+__global__ void k_global_malloc_combined_bulk(const uint64_t* sizes, uint64_t n, int on_host,
+                                              void** out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = global_malloc_combined(sizes[i], on_host != 0);
+}
+
+__global__ void k_global_free_combined_bulk(void* const* ptrs, uint64_t n, int on_host) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && ptrs[i]) global_free_combined(ptrs[i], on_host != 0);
+}
+
+__global__ void k_global_malloc_poison_bulk(const uint64_t* sizes, uint64_t n, void** out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = global_malloc_poison(sizes[i]);
+}
+
+__global__ void k_global_check_poison_bulk(void* const* ptrs, uint64_t n, uint8_t* ok) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ok[i] = global_check_poison(ptrs[i]) ? 1 : 0;
+}
+
+__global__ void k_global_free_poison_bulk(void* const* ptrs, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && ptrs[i]) global_free_poison(ptrs[i]);
+}
+
+static global_allocator_type* current_global_host() {
+  global_allocator_type* p = nullptr;
+  if (hipMemcpyFromSymbol(&p, HIP_SYMBOL(global_host_gallatin), sizeof(p)) != hipSuccess)
+    return nullptr;
+  return p;
+}
+
 static global_allocator_type* current_global() {
   global_allocator_type* p = nullptr;
   if (hipMemcpyFromSymbol(&p, HIP_SYMBOL(global_gallatin), sizeof(p)) != hipSuccess)
@@ -60,6 +103,100 @@ int gal_print_global_stats(void) {
 }
 
 gal_allocator* gal_global_handle(void) { return gal_from_device_state(current_global()); }
+
+int gal_init_global_host(uint64_t num_bytes, uint64_t seed, int print_info) {
+  if (current_global_host()) return fail(GAL_ERR_INVALID, "host allocator already initialised");
+  init_global_allocator_host(num_bytes, seed, print_info != 0, false);
+  if (!current_global_host()) return fail(GAL_ERR_BOOT, "init_global_allocator_host failed");
+  return GAL_OK;
+}
+
+int gal_free_global_host(void) {
+  if (!current_global_host()) return fail(GAL_ERR_INVALID, "no host allocator");
+  free_global_allocator_host();
+  return GAL_OK;
+}
+
+int gal_print_global_stats_host(void) {
+  if (!current_global_host()) return fail(GAL_ERR_INVALID, "no host allocator");
+  print_global_stats_host();
+  return GAL_OK;
+}
+
+gal_allocator* gal_global_host_handle(void) {
+  return gal_from_device_state(current_global_host());
+}
+
+static unsigned grid_of(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+int gal_global_malloc_fused_bulk(const uint64_t* sizes, uint64_t n, void** out, void* stream) {
+  if ((!sizes || !out) && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global() || !current_global_host())
+    return fail(GAL_ERR_INVALID, "fused malloc needs the global and the host allocator");
+  if (n == 0) return GAL_OK;
+  k_global_malloc_fused_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(sizes, n, out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_malloc_fused");
+}
+
+int gal_global_free_fused_bulk(void* const* ptrs, uint64_t n, void* stream) {
+  if (!ptrs && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global() || !current_global_host())
+    return fail(GAL_ERR_INVALID, "fused free needs the global and the host allocator");
+  if (n == 0) return GAL_OK;
+  k_global_free_fused_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(ptrs, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_free_fused");
+}
+
+int gal_global_malloc_combined_bulk(const uint64_t* sizes, uint64_t n, int on_host, void** out,
+                                    void* stream) {
+  if ((!sizes || !out) && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!(on_host ? current_global_host() : current_global()))
+    return fail(GAL_ERR_INVALID, "that allocator is not initialised");
+  if (n == 0) return GAL_OK;
+  k_global_malloc_combined_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(sizes, n, on_host,
+                                                                            out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_malloc_combined");
+}
+
+int gal_global_free_combined_bulk(void* const* ptrs, uint64_t n, int on_host, void* stream) {
+  if (!ptrs && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!(on_host ? current_global_host() : current_global()))
+    return fail(GAL_ERR_INVALID, "that allocator is not initialised");
+  if (n == 0) return GAL_OK;
+  k_global_free_combined_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(ptrs, n, on_host);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_free_combined");
+}
+
+int gal_global_malloc_poison_bulk(const uint64_t* sizes, uint64_t n, void** out, void* stream) {
+  if ((!sizes || !out) && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  if (n == 0) return GAL_OK;
+  k_global_malloc_poison_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(sizes, n, out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_malloc_poison");
+}
+
+int gal_global_check_poison_bulk(void* const* ptrs, uint64_t n, uint8_t* ok, void* stream) {
+  if ((!ptrs || !ok) && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  if (n == 0) return GAL_OK;
+  k_global_check_poison_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(ptrs, n, ok);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_check_poison");
+}
+
+int gal_global_free_poison_bulk(void* const* ptrs, uint64_t n, void* stream) {
+  if (!ptrs && n) return fail(GAL_ERR_INVALID, "null argument");
+  if (!current_global()) return fail(GAL_ERR_INVALID, "no global allocator");
+  if (n == 0) return GAL_OK;
+  k_global_free_poison_bulk<<<grid_of(n), 256, 0, (hipStream_t)stream>>>(ptrs, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, "global_free_poison");
+}
 
 int gal_global_malloc_bulk(const uint64_t* sizes, uint64_t n, void** out, uint32_t threads,
                            void* stream) {

@@ -16,6 +16,7 @@ struct gal_allocator {
   uint64_t meta_bytes = 0;
   void* pool = nullptr;
   uint64_t pool_bytes = 0;
+  int pool_kind = 0;                     // 0 device, 1 pinned host, 2 managed
   uint64_t* tree_base = nullptr;         // device address of the tree arena
   uint64_t tree_words = 0;
   int device = 0;

@@ -769,6 +769,27 @@ struct Gallatin : public gal_state {
   static __host__ my_type* generate_on_device(uint64_t max_bytes, uint64_t seed,
                                               bool print_info = true,
                                               bool running_calloc = false) {
+    return generate_with_pool(max_bytes, seed, print_info, running_calloc, 0u);
+  }
+
+  // gallatin.cuh:427-533: the pool in pinned, mapped host memory; the
+  // metadata stays in device memory.
+  static __host__ my_type* generate_on_device_host(uint64_t max_bytes, uint64_t seed,
+                                                   bool print_info = true,
+                                                   bool running_calloc = false) {
+    return generate_with_pool(max_bytes, seed, print_info, running_calloc, GAL_FLAG_POOL_HOST);
+  }
+
+  // gallatin.cuh:536-630: the pool in managed memory.
+  static __host__ my_type* generate_on_device_managed(uint64_t max_bytes, uint64_t seed,
+                                                      bool print_info = true,
+                                                      bool running_calloc = false) {
+    return generate_with_pool(max_bytes, seed, print_info, running_calloc,
+                              GAL_FLAG_POOL_MANAGED);
+  }
+
+  static __host__ my_type* generate_with_pool(uint64_t max_bytes, uint64_t seed, bool print_info,
+                                              bool running_calloc, uint32_t pool_flag) {
     if (running_calloc) {
       fprintf(stderr, "Gallatin: calloc mode is not supported\n");
       return nullptr;
@@ -779,6 +800,7 @@ struct Gallatin : public gal_state {
     cfg.smallest = Smallest;
     cfg.biggest = Biggest;
     cfg.print_info = print_info ? 1u : 0u;
+    cfg.flags |= pool_flag;
     gal_allocator* h = nullptr;
     if (gal_create(&cfg, &h) != GAL_OK) {
       fprintf(stderr, "Gallatin: generate_on_device failed: %s\n", gal_last_error());

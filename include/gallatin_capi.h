@@ -36,6 +36,13 @@ typedef enum gal_status {
 /* flags */
 #define GAL_FLAG_DETERMINISTIC 1u     /* smallest-id segments, slot scan from 0 */
 #define GAL_FLAG_NO_PAYLOAD_FENCE 2u  /* caller orders payload itself */
+/* Pool placement (metadata stays in device memory either way):
+ * GAL_FLAG_POOL_HOST: pinned, mapped host memory (generate_on_device_host,
+ * gallatin.cuh:427-533; memory_table.cuh:521-542); GAL_FLAG_POOL_MANAGED:
+ * managed memory (generate_on_device_managed, gallatin.cuh:536-630;
+ * memory_table.cuh:543-560).  At most one of the two. */
+#define GAL_FLAG_POOL_HOST 4u
+#define GAL_FLAG_POOL_MANAGED 8u
 
 typedef struct gal_config {
   uint64_t max_bytes;          /* pool bytes; segments = max_bytes / bytes_per_segment */
@@ -208,6 +215,37 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
 int gal_run_churn_batched(gal_allocator *a, uint64_t seed, uint64_t nthreads,
                           uint32_t rounds, uint64_t *counters_dev,
                           uint32_t workgroups, uint32_t threads, void *stream);
+
+/* ---- the host-pool global instance, combined, fused and poison ----
+ * (global_allocator.cuh:44, 100-357.)  gal_init_global_host and friends
+ * manage a second global instance, Gallatin<16 MiB, 16, 4096> with its pool
+ * in pinned host memory (init_global_allocator_host, :100-104;
+ * free_global_allocator_host, :106-112; print_global_stats_host, :123-131).
+ * Bulk forms of the device calls, one thread per element:
+ *  - fused (global_malloc_fused / global_free_fused, :195-216): the device
+ *    pool first, the host pool when it has no room; free goes to the owner;
+ *  - combined (global_malloc_combined / global_free_combined, :154-178):
+ *    on_host picks the instance;
+ *  - poison (global_malloc_poison / global_check_poison / global_free_poison,
+ *    :238-357) on the device instance: size records before and after the
+ *    allocation and a byte pattern in the slot's padding; check writes
+ *    ok[i] = 1 when they are intact. */
+int gal_init_global_host(uint64_t num_bytes, uint64_t seed, int print_info);
+int gal_free_global_host(void);
+int gal_print_global_stats_host(void);
+gal_allocator *gal_global_host_handle(void);
+int gal_global_malloc_fused_bulk(const uint64_t *sizes_dev, uint64_t n, void **out_dev,
+                                 void *stream);
+int gal_global_free_fused_bulk(void *const *ptrs_dev, uint64_t n, void *stream);
+int gal_global_malloc_combined_bulk(const uint64_t *sizes_dev, uint64_t n, int on_host,
+                                    void **out_dev, void *stream);
+int gal_global_free_combined_bulk(void *const *ptrs_dev, uint64_t n, int on_host,
+                                  void *stream);
+int gal_global_malloc_poison_bulk(const uint64_t *sizes_dev, uint64_t n, void **out_dev,
+                                  void *stream);
+int gal_global_check_poison_bulk(void *const *ptrs_dev, uint64_t n, uint8_t *ok_dev,
+                                 void *stream);
+int gal_global_free_poison_bulk(void *const *ptrs_dev, uint64_t n, void *stream);
 
 /* ---- allocator-backed data structures (on the global allocator) ----
  * gal_vector: fixed_vector<uint64_t, 64, 2^22> (data_structs/fixed_vector.cuh
