@@ -61,14 +61,13 @@ __device__ inline void flush_counts(uint64_t* ctr, const op_counts& c) {
 // only after it has seen the entry.
 constexpr uint64_t kMallocFailed = 1;
 
-// Decode op i: the pointer to free (taken out of the table, tag checked) or
-// the size to allocate.
-__device__ inline void take_op(const gal_batch_op* ops, uint64_t i, bool active,
-                               uint64_t* ptrs, uint64_t nslots, op_counts& c, void*& old,
-                               bool& want, uint64_t& size, uint32_t& slot,
-                               uint32_t spin_cap, uint64_t* ctr) {
-  old = nullptr;
+// Decode op i without waiting: a malloc (want, size) or a free (is_free)
+// of table entry `slot`.
+__device__ inline void decode_op(const gal_batch_op* ops, uint64_t i, bool active,
+                                 uint64_t nslots, op_counts& c, bool& want, uint64_t& size,
+                                 uint32_t& slot, bool& is_free) {
   want = false;
+  is_free = false;
   size = 0;
   slot = 0;
   if (!active) return;
@@ -81,13 +80,21 @@ __device__ inline void take_op(const gal_batch_op* ops, uint64_t i, bool active,
   if (op.kind == 0) {
     want = true;
     size = op.arg;
-    return;
+  } else {
+    is_free = true;
   }
+}
+
+// The pointer a free op releases: taken out of the table (bounded wait
+// for its malloc to publish it), tag checked; nullptr when the malloc
+// failed or the wait ran out.
+__device__ inline void* wait_free(uint64_t* ptrs, uint32_t slot, op_counts& c,
+                                  uint32_t spin_cap, uint64_t* ctr) {
   uint64_t v = utils::atomic_load(&ptrs[slot]);
   for (uint32_t spins = 0; v == 0; ++spins) {  // its malloc has not run yet
     if (spins >= spin_cap) {
       utils::atomic_add(&ctr[GAL_CTR_TIMEOUTS], (uint64_t)1);
-      return;
+      return nullptr;
     }
     __builtin_amdgcn_s_sleep(4);
     v = utils::atomic_load(&ptrs[slot]);
@@ -95,13 +102,24 @@ __device__ inline void take_op(const gal_batch_op* ops, uint64_t i, bool active,
   utils::atomic_store(&ptrs[slot], (uint64_t)0);
   if (v == kMallocFailed) {
     c.skipped += 1;
-    return;
+    return nullptr;
   }
   uint64_t* p = reinterpret_cast<uint64_t*>(v);
   c.bad += utils::atomic_load(p) != (uint64_t)slot;
   c.bytes += 16;  // tag written at malloc, read here
   c.pairs += 1;
-  old = p;
+  return p;
+}
+
+// Decode op i: the pointer to free (taken out of the table, tag checked) or
+// the size to allocate.
+__device__ inline void take_op(const gal_batch_op* ops, uint64_t i, bool active,
+                               uint64_t* ptrs, uint64_t nslots, op_counts& c, void*& old,
+                               bool& want, uint64_t& size, uint32_t& slot,
+                               uint32_t spin_cap, uint64_t* ctr) {
+  bool is_free = false;
+  decode_op(ops, i, active, nslots, c, want, size, slot, is_free);
+  old = is_free ? wait_free(ptrs, slot, c, spin_cap, ctr) : nullptr;
 }
 
 __device__ inline void give_op(void* np, bool want, uint32_t slot, uint64_t* ptrs,
@@ -219,7 +237,10 @@ __global__ __launch_bounds__(256) void k_table_free_plain(gal_state* s, uint64_t
 // run them at once: a free waits only for its own table entry.  A chunk is
 // only taken by a running workgroup and a free's malloc lies in a chunk
 // taken earlier, so the earliest unfinished chunk never waits and the grid
-// always drains; the waits are bounded all the same (counter [5]).
+// always drains; the waits are bounded all the same (counter [5]).  Each
+// round of a chunk serves and publishes its mallocs before it waits on its
+// frees, which shortens the chain of waits through the stream; the batched
+// form batches the mallocs across the workgroup and frees per wave.
 struct stream_chunk {
   uint64_t begin;  // first op
   uint32_t count;  // ops
@@ -248,20 +269,22 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     const uint32_t rounds = (ch.count + kThreads - 1) / kThreads;
     for (uint32_t r = 0; r < rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
-      void* old;
-      bool want;
+      bool want, is_free;
       uint64_t size;
       uint32_t slot;
-      take_op(ops, ch.begin + j, j < ch.count, ptrs, nslots, c, old, want, size, slot,
-              spin_cap, ctr);
+      decode_op(ops, ch.begin + j, j < ch.count, nslots, c, want, size, slot, is_free);
+      // Mallocs first, published before this round waits on any free: the
+      // frees of later chunks wait on exactly these entries.
       void* np = nullptr;
-      if (kBatched) {
-        np = batch::batch_step(s, old, want, size, lds);
-      } else {
-        if (old) detail::free_impl(s, old);
-        if (want) np = detail::malloc_impl(s, size);
-      }
+      if (kBatched)
+        np = batch::batch_malloc(s, want, size, lds);
+      else if (want)
+        np = detail::malloc_impl(s, size);
       give_op(np, want, slot, ptrs, c);
+      // Frees go through the wave path in both forms: a workgroup-wide
+      // free would hold every wave behind the slowest wait.
+      void* old = is_free ? wait_free(ptrs, slot, c, spin_cap, ctr) : nullptr;
+      if (old) detail::free_impl(s, old);
     }
   }
   flush_counts(ctr, c);
