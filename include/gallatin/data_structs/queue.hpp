@@ -15,10 +15,15 @@
 // address, and each update bumps the count, so a stale compare-and-swap
 // fails.  A node another dequeuer freed may still be read before such a
 // compare-and-swap fails; its memory stays in the allocator's pool (it is
-// never unmapped), and an address read from it is used only if it lies in
-// the pool.  Node words are read and written with device-coherent atomics,
-// since producers and consumers may run on different XCDs.  Every retry
-// loop is bounded; enqueue reports failure (no node, or no progress).
+// never unmapped), and an address read from it is used only if it could be
+// a node of the pool.  Node words are read and written with device-coherent
+// atomics, since producers and consumers may run on different XCDs.  Every
+// retry loop is bounded; enqueue reports failure (no node, or no progress).
+//
+// Lanes of one wave that work on the same queue act as one: their enqueues
+// append one chain of nodes with one compare-and-swap, and their dequeues
+// move `head` past as many nodes with one compare-and-swap, where the
+// reference makes every lane compete for the same word.
 #ifndef GALLATIN_QUEUE_HPP
 #define GALLATIN_QUEUE_HPP
 
@@ -131,65 +136,150 @@ struct queue {
     head = tail = 0;
   }
 
+  // An address that may be a node: inside the pool, with room for a whole
+  // node, and aligned as every slice is (16 bytes).
   __device__ __forceinline__ bool in_pool(uint64_t a) const {
-    return a >= pool_lo && a < pool_hi;
+    return a >= pool_lo && a <= pool_hi - sizeof(node) && (a & 15) == 0;
   }
 
   // queue.cuh:90-135.  False when no node could be allocated or no progress
-  // was made within the retry bound.
+  // was made within the retry bound.  The lanes of a wave that enqueue into
+  // this queue together link their nodes into one chain in lane order, and
+  // one lane appends the chain with a single compare-and-swap.
   __device__ bool try_enqueue(T item) {
     node* nd = static_cast<node*>(my_backing_allocator->malloc(sizeof(node)));
-    if (!nd) return false;
-    nd->store_item(item);
-    gallatin::utils::atomic_store(&nd->next, (uint64_t)0);
-    __builtin_amdgcn_s_waitcnt(0);  // the node is complete before it is linked
-    for (uint32_t tries = 0; tries < kTries; ++tries) {
-      const uint64_t t = gallatin::utils::atomic_load(&tail);
-      node* tn = reinterpret_cast<node*>(tagged::addr(t));
-      const uint64_t n = gallatin::utils::atomic_load(&tn->next);
-      if (t != gallatin::utils::atomic_load(&tail)) continue;
-      if (tagged::addr(n) == 0) {
-        const uint64_t linked = tagged::make((uint64_t)nd, tagged::count(n) + 1);
-        if (gallatin::utils::atomic_cas(&tn->next, n, linked) == n) {
-          gallatin::utils::atomic_cas(&tail, t, tagged::make((uint64_t)nd, tagged::count(t) + 1));
-          return true;
-        }
-      } else if (in_pool(tagged::addr(n))) {  // tail is behind: help it along
-        gallatin::utils::atomic_cas(&tail, t, tagged::make(tagged::addr(n), tagged::count(t) + 1));
-      }
-      __builtin_amdgcn_s_sleep(1);
+    if (nd) {
+      nd->store_item(item);
+      gallatin::utils::atomic_store(&nd->next, (uint64_t)0);
     }
-    my_backing_allocator->free(nd);
-    return false;
+    const bool have = nd != nullptr;
+    const uint64_t group = gallatin::wave::match(__ballot(have), have, (uint64_t)this);
+    if (!have) return false;
+    const uint32_t lane = gallatin::wave::lane();
+    const uint64_t above = group & ~((2ULL << lane) - 1);  // group lanes after this one
+    const int leader = gallatin::wave::first(group);
+    const int last = 63 - __builtin_clzll(group);
+    const uint64_t succ = gallatin::wave::fetch((uint64_t)nd, above ? gallatin::wave::first(above) : (int)lane);
+    if (above) gallatin::utils::atomic_store(&nd->next, tagged::make(succ, 0));
+    __builtin_amdgcn_s_waitcnt(0);  // every node of the chain is complete before it is linked
+    const uint64_t first_node = gallatin::wave::fetch((uint64_t)nd, leader);
+    const uint64_t last_node = gallatin::wave::fetch((uint64_t)nd, last);
+    uint32_t linked = 0;
+    if ((int)lane == leader) {
+      for (uint32_t tries = 0; tries < kTries; ++tries) {
+        const uint64_t t = gallatin::utils::atomic_load(&tail);
+        node* tn = reinterpret_cast<node*>(tagged::addr(t));
+        const uint64_t n = gallatin::utils::atomic_load(&tn->next);
+        if (t != gallatin::utils::atomic_load(&tail)) continue;
+        if (tagged::addr(n) == 0) {
+          const uint64_t link = tagged::make(first_node, tagged::count(n) + 1);
+          if (gallatin::utils::atomic_cas(&tn->next, n, link) == n) {
+            gallatin::utils::atomic_cas(&tail, t, tagged::make(last_node, tagged::count(t) + 1));
+            linked = 1;
+            break;
+          }
+        } else if (in_pool(tagged::addr(n))) {  // tail is behind: help it along
+          gallatin::utils::atomic_cas(&tail, t,
+                                      tagged::make(tagged::addr(n), tagged::count(t) + 1));
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    linked = gallatin::wave::fetch(linked, leader);
+    if (!linked) my_backing_allocator->free(nd);
+    return linked != 0;
   }
 
   __device__ void enqueue(T item) { (void)try_enqueue(item); }
 
-  // queue.cuh:139-187.  False when the queue is empty.
+  // queue.cuh:139-187.  False when the queue is empty.  The lanes of a wave
+  // that dequeue from this queue together walk up to that many nodes from
+  // `head` in step, read the items of those nodes, then move `head` past
+  // all of them with one compare-and-swap; lane r keeps the item of the
+  // r-th node, and the nodes that stop being the dummy are freed.  Items
+  // are read before the compare-and-swap, as in the one-node form: the last
+  // node taken becomes the dummy, which a later dequeue may free at once.
   __device__ bool dequeue(T& return_val) {
-    for (uint32_t tries = 0; tries < kTries; ++tries) {
-      const uint64_t h = gallatin::utils::atomic_load(&head);
-      const uint64_t t = gallatin::utils::atomic_load(&tail);
-      node* hn = reinterpret_cast<node*>(tagged::addr(h));
-      const uint64_t n = gallatin::utils::atomic_load(&hn->next);
-      if (h != gallatin::utils::atomic_load(&head)) continue;
-      const uint64_t na = tagged::addr(n);
-      if (tagged::addr(h) == tagged::addr(t)) {
-        if (na == 0) return false;  // empty
-        if (in_pool(na))
-          gallatin::utils::atomic_cas(&tail, t, tagged::make(na, tagged::count(t) + 1));
+    const uint64_t group = gallatin::wave::match(__ballot(1), true, (uint64_t)this);
+    const uint32_t lane = gallatin::wave::lane();
+    const int leader = gallatin::wave::first(group);
+    const uint32_t k = gallatin::wave::count(group);
+    const uint32_t rank = gallatin::wave::count(group & gallatin::wave::lanes_below());
+    // A wave may hold groups of several queues: each group runs the loop
+    // below with values broadcast from its leader, so every branch is
+    // uniform within the group.
+    uint64_t mine = 0;   // the node whose item this lane takes
+    uint32_t got = 0;    // nodes taken by the group (0 = empty)
+    uint64_t old_head = 0;
+    T val{};
+    bool done = false;
+    for (uint32_t tries = 0; tries < kTries && !done; ++tries) {
+      const uint64_t h = gallatin::wave::fetch(gallatin::utils::atomic_load(&head), leader);
+      const uint64_t t = gallatin::wave::fetch(gallatin::utils::atomic_load(&tail), leader);
+      // walk up to k nodes after the dummy
+      uint64_t cur = tagged::addr(h);
+      uint32_t m = 0;
+      bool stale = false;
+      mine = 0;
+      for (uint32_t i = 0; i < k; ++i) {
+        const uint64_t nx = gallatin::wave::fetch(
+            gallatin::utils::atomic_load(&reinterpret_cast<node*>(cur)->next), leader);
+        const uint64_t na = tagged::addr(nx);
+        if (na == 0) break;
+        if (!in_pool(na)) {  // read from a node already recycled
+          stale = true;
+          break;
+        }
+        if (rank == i) mine = na;
+        cur = na;
+        m = i + 1;
+      }
+      const uint64_t h2 = gallatin::wave::fetch(gallatin::utils::atomic_load(&head), leader);
+      if (stale || h2 != h) continue;
+      if (m == 0) {
+        if (tagged::addr(h) == tagged::addr(t)) {  // empty
+          done = true;
+          break;
+        }
         continue;
       }
-      if (!in_pool(na)) continue;  // read from a node already recycled
-      const T v = reinterpret_cast<node*>(na)->load_item();
-      if (gallatin::utils::atomic_cas(&head, h, tagged::make(na, tagged::count(h) + 1)) == h) {
-        return_val = v;
-        my_backing_allocator->free(hn);
-        return true;
+      // tail must not stay on a node about to be freed: the dummy or any
+      // taken node but the last (which becomes the dummy)
+      const uint64_t ta = tagged::addr(t);
+      const bool tail_in =
+          ta == tagged::addr(h) || (__ballot(rank + 1 < m && mine == ta) & group) != 0;
+      if (tail_in) {
+        if ((int)lane == leader) {
+          const uint64_t tn = gallatin::utils::atomic_load(&reinterpret_cast<node*>(ta)->next);
+          if (in_pool(tagged::addr(tn)))
+            gallatin::utils::atomic_cas(&tail, t,
+                                        tagged::make(tagged::addr(tn), tagged::count(t) + 1));
+        }
+        continue;
       }
-      __builtin_amdgcn_s_sleep(1);
+      if (rank < m) val = reinterpret_cast<node*>(mine)->load_item();
+      uint32_t won = 0;
+      if ((int)lane == leader)
+        won = gallatin::utils::atomic_cas(&head, h, tagged::make(cur, tagged::count(h) + 1)) == h;
+      won = gallatin::wave::fetch(won, leader);
+      if (won) {
+        got = m;
+        old_head = tagged::addr(h);
+        done = true;
+      } else {
+        __builtin_amdgcn_s_sleep(1);
+      }
     }
-    return false;
+    const bool take = rank < got;
+    if (take) return_val = val;
+    // the node before each taken one stops being the dummy: lane 0 of the
+    // group frees the old dummy, lane r frees the node taken by lane r - 1
+    const uint64_t below = group & gallatin::wave::lanes_below();
+    const int prev = below ? 63 - __builtin_clzll(below) : (int)lane;
+    const uint64_t prev_node = gallatin::wave::fetch(mine, prev);
+    __builtin_amdgcn_s_waitcnt(0);  // items are read before their nodes are freed
+    if (take) my_backing_allocator->free(reinterpret_cast<node*>(rank == 0 ? old_head : prev_node));
+    return take;
   }
 };
 

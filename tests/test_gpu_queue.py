@@ -27,7 +27,7 @@ def test_queue_parallel_roundtrip(gpu):
     import torch
     a = _alloc(gpu)
     q = gpu.Queue(a)
-    n = 1 << 18
+    n = 1 << 20
     items = torch.arange(1, n + 1, dtype=torch.int64, device="cuda")
     ok = q.enqueue(items)
     out, got = q.dequeue(n)
@@ -60,7 +60,7 @@ def test_queue_concurrent_producers_and_consumers(gpu):
     import torch
     a = _alloc(gpu)
     q = gpu.Queue(a)
-    n = 1 << 17
+    n = 1 << 19
     items = torch.arange(1, n + 1, dtype=torch.int64, device="cuda")
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
     with torch.cuda.stream(s1):
