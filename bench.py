@@ -142,11 +142,14 @@ def hbm_ceiling(G, torch):
     return out
 
 
-def plain_small(args, G, torch):
+def plain_small(args, G, torch, flags=0):
     """Plain method, small: 1 GiB pool, Gallatin<16 MiB, 16, 4096>, one
     launch of 2^20 threads each doing malloc(16), write its uint32 id, read
-    it back, free; 10 launches per step."""
-    a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(1 << 30, args.seed, print_info=False)
+    it back, free; 10 launches per step. With GAL_FLAG_NO_PAYLOAD_FENCE the
+    id is written and read with device-coherent accesses and the allocator
+    skips its payload fences."""
+    a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(1 << 30, args.seed, print_info=False,
+                                                          flags=flags)
     ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
     n, launches, steps = 1 << 20, 10, 5
     for _ in range(launches):  # warm-up step
@@ -395,6 +398,11 @@ def main():
             hbm["traffic_frac_of_scatter_2048"] = hbm["traffic_gbs"] / hbm["scatter_2048_gbs"]
         out["hbm_ceilings"] = hbm
     out["plain_small"] = plain_small(args, G, torch)
+    # the same workload with the id written and read device-coherently, so
+    # the allocator needs no payload fences (an L2 write-back per free on a
+    # pool too small for XCD partitions)
+    out["plain_small"]["coherent_payload"] = plain_small(
+        args, G, torch, flags=G.GAL_FLAG_NO_PAYLOAD_FENCE)
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)
     if not args.no_serving:

@@ -181,18 +181,28 @@ __global__ void k_wave_stream(gal_state* s, const gal_wave_step* steps, uint64_t
   }
 }
 
-// Plain method: malloc(size), write the thread id, read it back, free.
+// Plain method: malloc(size), write the thread id, read it back, free.  An
+// instance created with GAL_FLAG_NO_PAYLOAD_FENCE leaves the ordering of
+// payload to the caller, so there the id is written and read with
+// device-coherent accesses (they do not stay in one XCD's L2).
 __global__ void k_plain(gal_state* s, uint64_t n, uint64_t size, uint64_t* ctr) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool coherent = (s->flags & kFlagNoPayloadFence) != 0;
   lane_counts c;
   if (tid < n) {
-    volatile uint32_t* p = static_cast<volatile uint32_t*>(detail::malloc_impl(s, size));
+    uint32_t* p = static_cast<uint32_t*>(detail::malloc_impl(s, size));
     c.mallocs = 1;
     if (!p) {
       c.misses = 1;
     } else {
-      p[0] = (uint32_t)tid;
-      c.bad = p[0] != (uint32_t)tid;
+      if (coherent) {
+        utils::atomic_store(p, (uint32_t)tid);
+        c.bad = utils::atomic_load(p) != (uint32_t)tid;
+      } else {
+        volatile uint32_t* v = p;
+        v[0] = (uint32_t)tid;
+        c.bad = v[0] != (uint32_t)tid;
+      }
       c.bytes = 8;
       detail::free_impl(s, (void*)p);
       c.pairs = 1;

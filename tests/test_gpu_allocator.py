@@ -148,11 +148,15 @@ def test_class_recovers_after_running_out(gpu):
     a.free_on_device()
 
 
-def test_plain_small_config(gpu):
+@pytest.mark.parametrize("coherent", [False, True])
+def test_plain_small_config(gpu, coherent):
     """1 GiB pool, 2^20 threads x 10 launches of malloc(16) / write id /
-    read back / free; every pair succeeds and the pool returns to fresh."""
+    read back / free; every pair succeeds and the pool returns to fresh.
+    coherent: the instance skips its payload fences and the id is written
+    and read with device-coherent accesses."""
     import torch
-    a = make(gpu, 1 << 30, det=False, limits=None)
+    flags = gpu.GAL_FLAG_NO_PAYLOAD_FENCE if coherent else 0
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(1 << 30, 42, print_info=False, flags=flags)
     ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
     for _ in range(10):
         a.run_plain(1 << 20, 16, counters=ctr)
