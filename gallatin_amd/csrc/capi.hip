@@ -274,22 +274,28 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   h.lim_slice = cfg->slice_attempts ? cfg->slice_attempts : (uint32_t)kSliceAttempts;
   h.lim_block = cfg->block_attempts ? cfg->block_attempts : (uint32_t)kBlockAttempts;
   h.lim_segment = cfg->segment_attempts ? cfg->segment_attempts : (uint32_t)kSegmentAttempts;
-  // XCD partitions: one per XCD (32 CUs each on MI355X) unless deterministic
+  // XCD partitions: one per XCD (32 CUs each on MI355X) or none.  A free
+  // into a segment of its own partition only releases its payload to its
+  // XCD's L2, which is sound only when a partition is exactly one XCD: with
+  // two XCDs in a partition, a slice freed on one could be reused on the
+  // other while the first XCD's L2 still holds the old payload as dirty
+  // lines.  So P is the XCD count or 1.  Booting pins blocks of every class
+  // in every partition; a pool where that would take more than about a
+  // quarter of its segments gets one partition.
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, a->device) != hipSuccess)
+    cus = 32;
+  const uint32_t xcds = cus >= 64 ? (uint32_t)(cus / 32) : 1u;
   uint32_t P = cfg->partitions;
   if (cfg->flags & kFlagDeterministic) {
     P = 1;
   } else if (P == 0) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, a->device) != hipSuccess)
-      cus = 32;
-    P = (uint32_t)(cus / 32);
-    // each class pins blocks in every partition at boot: keep that to about
-    // a quarter of the pool
-    const uint64_t fit = N / (4ULL * T);
-    if (P > fit) P = (uint32_t)fit;
+    P = (uint64_t)xcds <= N / (4ULL * T) ? xcds : 1u;
+  } else if (P != 1 && P != xcds) {
+    delete a;
+    return fail(GAL_ERR_INVALID, "partitions must be 0 (automatic), 1 or the number of XCDs");
   }
-  if (P < 1) P = 1;
-  if (P > (uint32_t)kMaxParts) P = kMaxParts;
+  if (P < 1 || P > (uint32_t)kMaxParts) P = 1;  // a partition per XCD, or one
   h.num_parts = P;
   // pinned slot counts, shared_block_storage.cuh:251-274, each rounded up to
   // a multiple of the partitions

@@ -148,6 +148,23 @@ def test_class_recovers_after_running_out(gpu):
     a.free_on_device()
 
 
+def test_partitions_are_one_or_per_xcd(gpu):
+    """A partition must be exactly one XCD (a home free only releases its
+    payload to its own XCD's L2), so an instance has one partition or one
+    per XCD: 8 GiB gets one per XCD, 4 GiB (too small to pin every class in
+    8 partitions) gets one, and any other explicit count is refused."""
+    import torch
+    xcds = torch.cuda.get_device_properties(0).multi_processor_count // 32
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(8 << 30, 1, print_info=False)
+    assert a.partitions == xcds
+    a.free_on_device()
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(4 << 30, 1, print_info=False)
+    assert a.partitions == 1
+    a.free_on_device()
+    with pytest.raises(gpu.GallatinError):
+        gpu.Gallatin(S, 16, 4096).generate_on_device(8 << 30, 1, print_info=False, partitions=4)
+
+
 @pytest.mark.parametrize("coherent", [False, True])
 def test_plain_small_config(gpu, coherent):
     """1 GiB pool, 2^20 threads x 10 launches of malloc(16) / write id /
