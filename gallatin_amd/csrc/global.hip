@@ -388,13 +388,15 @@ int gal_table_stats(gal_table* t, uint64_t* keys, uint64_t* buckets, uint64_t* l
                     uint64_t* failed) {
   if (!t) return fail(GAL_ERR_INVALID, "null argument");
   if (hipDeviceSynchronize() != hipSuccess) return fail(GAL_ERR_HIP, "sync");
-  uint64_t hdr[5] = {0, 0, 0, 0, 0};  // directory, level, seed, buckets, failed
+  uint64_t hdr[5] = {0, 0, 0, 0, 0};  // directory, level, seed, reserved, failed
   if (hipMemcpy(hdr, t->dev, sizeof(hdr), hipMemcpyDeviceToHost) != hipSuccess)
     return fail(GAL_ERR_HIP, "stats copy");
-  if (keys)
-    *keys = t->kind == 0 ? table_small::count_keys(static_cast<table_small*>(t->dev))
-                         : table_large::count_keys(static_cast<table_large*>(t->dev));
-  if (buckets) *buckets = hdr[3];
+  uint64_t nb = 0;
+  const uint64_t nk = t->kind == 0
+                          ? table_small::count_keys(static_cast<table_small*>(t->dev), &nb)
+                          : table_large::count_keys(static_cast<table_large*>(t->dev), &nb);
+  if (keys) *keys = nk;
+  if (buckets) *buckets = nb;
   if (level) *level = hdr[1];
   if (failed) *failed = hdr[4];
   return GAL_OK;

@@ -65,12 +65,24 @@ __global__ void ext_ht_free_directory(table* t) {
   if (blockIdx.x == 0 && threadIdx.x == 0) t->free_directory();
 }
 
+// Keys present (out[0]) and buckets (out[1]): a bucket is the one directory
+// entry at its home.  One atomic per wave and counter.
 template <typename table>
 __global__ void ext_ht_count(table* t, uint64_t n, unsigned long long* out) {
   const uint64_t i = gallatin::utils::get_tid();
+  uint64_t keys = 0, buckets = 0;
   if (i < n) {
-    const uint64_t c = t->bucket_fill(i);
-    if (c) atomicAdd(out, (unsigned long long)c);
+    keys = t->bucket_fill(i);
+    buckets = t->directory && t->directory[i] ? 1 : 0;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int src = (int)(gallatin::wave::lane() ^ (uint32_t)off);
+    keys += gallatin::wave::fetch(keys, src);
+    buckets += gallatin::wave::fetch(buckets, src);
+  }
+  if (gallatin::wave::lane() == 0) {
+    if (keys) atomicAdd(out, (unsigned long long)keys);
+    if (buckets) atomicAdd(out + 1, (unsigned long long)buckets);
   }
 }
 
@@ -97,7 +109,7 @@ struct extendible_hash_table {
   bucket** directory;  // max_items entries
   uint64_t level;      // deepest bucket depth
   uint64_t seed;
-  uint64_t buckets;    // buckets in use
+  uint64_t reserved;   // (buckets are counted by count_keys)
   uint64_t failed;     // inserts refused (table full or no memory)
 
   // ---- host ----
@@ -134,16 +146,18 @@ struct extendible_hash_table {
     (void)hipFree(dev);
   }
 
-  // Keys present (calculate_fill's count, extendible_ht.cuh:1379-1411).
-  static __host__ uint64_t count_keys(my_type* dev) {
+  // Keys present (calculate_fill's count, extendible_ht.cuh:1379-1411) and,
+  // into *buckets, the buckets in use.
+  static __host__ uint64_t count_keys(my_type* dev, uint64_t* buckets = nullptr) {
     unsigned long long* c = nullptr;
-    if (hipMalloc(&c, sizeof(*c)) != hipSuccess) return 0;
-    (void)hipMemset(c, 0, sizeof(*c));
+    if (hipMalloc(&c, 2 * sizeof(*c)) != hipSuccess) return 0;
+    (void)hipMemset(c, 0, 2 * sizeof(*c));
     ext_ht_count<my_type><<<(unsigned)((max_items + 255) / 256), 256>>>(dev, max_items, c);
-    unsigned long long n = 0;
-    (void)hipMemcpy(&n, c, sizeof(n), hipMemcpyDeviceToHost);
+    unsigned long long n[2] = {0, 0};
+    (void)hipMemcpy(n, c, sizeof(n), hipMemcpyDeviceToHost);
     (void)hipFree(c);
-    return n;
+    if (buckets) *buckets = n[1];
+    return n[0];
   }
 
   // ---- device: set-up and teardown ----
@@ -161,7 +175,6 @@ struct extendible_hash_table {
     gallatin::utils::atomic_store(&b->home, home);
     for (int i = 0; i < items_per_bucket; ++i)
       gallatin::utils::atomic_store(&b->keys[i], (uint64_t)defaultKey);
-    gallatin::utils::atomic_add(&buckets, (uint64_t)1);
     return b;
   }
 
@@ -227,9 +240,15 @@ struct extendible_hash_table {
   // fewer bits when that home has no bucket yet.  Entries are only ever
   // set, so the deepest one present among the loaded depths is the owner
   // (or a bucket being split, which the callers' home check rejects).
+  //
+  // `fresh` = false reads `level` through the cache: it may be behind, which
+  // only leads to an ancestor of the owner, and every caller checks the
+  // bucket's home against its depth and looks again (with fresh = true).
+  // Every operation reads `level`, and device-coherent reads of one word
+  // from the whole device queue at one memory channel.
   static constexpr int kOwnerFan = 4;
-  __device__ bucket* owner(uint64_t h) {
-    uint64_t top = ld(&level);
+  __device__ bucket* owner(uint64_t h, bool fresh) {
+    uint64_t top = fresh ? ld(&level) : *reinterpret_cast<volatile uint64_t*>(&level);
     for (;;) {
       uint64_t e[kOwnerFan];
 #pragma unroll
@@ -277,7 +296,6 @@ struct extendible_hash_table {
     const uint64_t home = old_home | (1ULL << d);
     bucket* nb = static_cast<bucket*>(gallatin::allocators::global_malloc(sizeof(bucket)));
     if (!nb) return false;
-    gallatin::utils::atomic_add(&buckets, (uint64_t)1);
     uint64_t moved = 0;
 #pragma unroll
     for (int s = 0; s < items_per_bucket; ++s) {
@@ -291,7 +309,9 @@ struct extendible_hash_table {
     st(&nb->home, home);
     __builtin_amdgcn_s_waitcnt(0);  // the new bucket is whole before it is published
     st(reinterpret_cast<uint64_t*>(&directory[home]), reinterpret_cast<uint64_t>(nb));
-    gallatin::utils::atomic_max(&level, (uint64_t)(d + 1));
+    // level only grows: a cached value at or above d + 1 needs no update
+    if (*reinterpret_cast<volatile uint64_t*>(&level) < d + 1)
+      gallatin::utils::atomic_max(&level, (uint64_t)(d + 1));
     __builtin_amdgcn_s_waitcnt(0);
     st(&b->ctl, ((uint64_t)(d + 1) << 32) | 1);  // still locked
     __builtin_amdgcn_s_waitcnt(0);  // new depth visible before keys leave
@@ -309,7 +329,7 @@ struct extendible_hash_table {
     const uint64_t h = hash(key);
     int state = 0;  // 0 pending, 1 stored, 2 refused
     for (uint32_t tries = 0; state == 0 && tries < kTries; ++tries) {
-      bucket* b = owner(h);
+      bucket* b = owner(h, tries > 0);
       if (!b) {
         state = 2;
         break;
@@ -350,7 +370,7 @@ struct extendible_hash_table {
   __device__ bool query(Key key, Val& val) {
     const uint64_t h = hash(key);
     for (uint32_t tries = 0; tries < kTries; ++tries) {
-      bucket* b = owner(h);
+      bucket* b = owner(h, tries > 0);
       if (!b) return false;
       const uint32_t d0 = depth_of(ld(&b->ctl));
       int match, empty;
@@ -378,7 +398,7 @@ struct extendible_hash_table {
     const uint64_t h = hash(key);
     int state = 0;  // 0 pending, 1 removed, 2 absent
     for (uint32_t tries = 0; state == 0 && tries < kTries; ++tries) {
-      bucket* b = owner(h);
+      bucket* b = owner(h, tries > 0);
       if (!b) break;
       uint32_t d;
       if (!try_lock(b, d)) {
