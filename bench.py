@@ -142,6 +142,28 @@ def hbm_ceiling(G, torch):
     return out
 
 
+def tree_probe(args, G, torch):
+    """Wave-cooperative successor searches on a fresh 64 GiB instance's
+    trees (3 levels, 4096 segments): alternately the segment tree (dense)
+    and class tree 0 (sparse: its boot segments only), from pseudo-random
+    starts, read only. 4096 waves x 2000 searches."""
+    a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(64 << 30, args.seed, print_info=False)
+    waves, searches = 4096, 2000
+    G.bench_tree(a, 0, 100, waves)  # warm
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    G.bench_tree(a, 0, searches, waves)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    a.free_on_device()
+    return {"workload": "64 GiB instance, 4096 waves x 2000 find_ge, segment tree and "
+                        "class tree 0 alternately",
+            "searches_per_s": waves * searches / (ms / 1e3),
+            "us_per_search_per_wave": ms * 1e3 / searches}
+
+
 def plain_small(args, G, torch, flags=0):
     """Plain method, small: 1 GiB pool, Gallatin<16 MiB, 16, 4096>, one
     launch of 2^20 threads each doing malloc(16), write its uint32 id, read
@@ -397,6 +419,7 @@ def main():
             hbm["traffic_frac_of_copy"] = hbm["traffic_gbs"] / hbm["copy_gbs"]
             hbm["traffic_frac_of_scatter_2048"] = hbm["traffic_gbs"] / hbm["scatter_2048_gbs"]
         out["hbm_ceilings"] = hbm
+        out["tree_search"] = tree_probe(args, G, torch)
     out["plain_small"] = plain_small(args, G, torch)
     # the same workload with the id written and read device-coherently, so
     # the allocator needs no payload fences (an L2 write-back per free on a

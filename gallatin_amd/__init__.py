@@ -45,7 +45,7 @@ CAPI_SYMBOLS = [
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
     "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
-    "gal_bench_scatter",
+    "gal_bench_scatter", "gal_bench_tree",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
     "gal_batch_plan_launch", "gal_batch_plan_destroy",
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
@@ -132,6 +132,7 @@ def lib():
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
         L.gal_bench_scatter.argtypes = [_p, _p, _u64, _u64, _p]
+        L.gal_bench_tree.argtypes = [_p, _u32, _u32, _u32, _p, _p]
         L.gal_run_batches.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32, _u32, _p]
         L.gal_batch_plan_create.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32,
                                             _u32, ctypes.POINTER(_p)]
@@ -778,6 +779,12 @@ def bench_atomic(words, span, n_ops, lanes_per_address=1, stride=1):
 
 def bench_copy(src, dst, nbytes):
     _check(lib().gal_bench_copy(src.data_ptr(), dst.data_ptr(), nbytes, _stream()))
+
+
+def bench_tree(alloc, tree, searches, waves):
+    sink = _dev_u64(1)
+    _check(lib().gal_bench_tree(alloc.handle, tree, searches, waves, sink.data_ptr(),
+                                _stream()))
 
 
 def bench_scatter(src, dst, nbytes, piece_bytes):

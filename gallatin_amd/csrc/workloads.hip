@@ -316,6 +316,26 @@ __global__ void k_bench_atomic(uint32_t* words, uint64_t span, uint64_t n_ops,
   if (acc == 0xFFFFFFFFu) words[0] = acc;
 }
 
+// Tree-search probe: every wave runs `searches` wave-cooperative find_ge
+// calls from pseudo-random starts, alternating between the instance's
+// segment tree and class tree `tree` of partition 0.  Read only: the trees
+// do not change, so counters over this kernel see the allocator's tree
+// reads alone.
+__global__ void k_bench_tree(const gal_state* s, uint32_t tree, uint32_t searches,
+                             uint64_t* sink) {
+  const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const gal_tree& seg = s->seg_tree;
+  const gal_tree& cls = s->sub_trees[tree * s->num_parts];
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < searches; ++i) {
+    const gal_tree& t = (i & 1) ? cls : seg;
+    uint64_t h = (wid * 0x9E3779B97F4A7C15ULL) ^ ((uint64_t)i * 0xC2B2AE3D27D4EB4FULL);
+    h ^= h >> 29;
+    acc += veb::find_ge(t, h % t.universe);
+  }
+  if (acc == 0x5DEECE66DULL) sink[0] = acc;
+}
+
 // Each workgroup copies whole tiles of kCopyVecs * blockDim.x 16-byte
 // vectors (a contiguous 64 KiB at 512 threads), tiles dealt round robin;
 // the vectors past the last whole tile go grid-stride.  Contiguous
@@ -486,6 +506,15 @@ int gal_bench_copy(const void* src, void* dst, uint64_t bytes, void* stream) {
   k_bench_copy<<<(unsigned)cus * 8, kCopyThreads, 0, (hipStream_t)stream>>>(
       static_cast<const u64x2*>(src), static_cast<u64x2*>(dst), bytes / 16);
   return launch_status("bench_copy");
+}
+
+int gal_bench_tree(gal_allocator* a, uint32_t tree, uint32_t searches, uint32_t waves,
+                   uint64_t* sink_dev, void* stream) {
+  if (!a || !sink_dev || waves == 0) return fail(GAL_ERR_INVALID, "bad tree probe arguments");
+  if (tree >= a->host.num_trees) return fail(GAL_ERR_INVALID, "no such class tree");
+  const unsigned blocks = (waves + 3) / 4;  // four waves per workgroup
+  k_bench_tree<<<blocks, 256, 0, (hipStream_t)stream>>>(a->dev, tree, searches, sink_dev);
+  return launch_status("bench_tree");
 }
 
 int gal_bench_scatter(const void* src, void* dst, uint64_t bytes, uint64_t piece_bytes,

@@ -417,6 +417,12 @@ int gal_bench_atomic(uint32_t *words_dev, uint64_t span, uint64_t n_ops,
  * workgroup streaming contiguous 64 KiB tiles. */
 int gal_bench_copy(const void *src_dev, void *dst_dev, uint64_t bytes,
                    void *stream);
+/* Tree-search probe: about `waves` waves (rounded up to whole workgroups
+ * of four) each run `searches` wave-cooperative successor searches from
+ * pseudo-random starts, alternating between the instance's segment tree and
+ * class tree `tree` of partition 0 (veb.cuh:878-916 search, read only). */
+int gal_bench_tree(gal_allocator *a, uint32_t tree, uint32_t searches, uint32_t waves,
+                   uint64_t *sink_dev, void *stream);
 /* Copy of `bytes` cut into pieces of `piece_bytes` (both powers of two,
  * piece >= 16 B), each piece read from and written to pseudo-randomly
  * permuted positions: the bandwidth ceiling of payload traffic spread over
