@@ -792,7 +792,7 @@ __global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t*
   if (i == 0) *top = (int32_t)n;
 }
 
-// Session pass: waves take 64 slots at a time from the queue.
+// Session pass: each wave takes 64 session slots at a time.
 __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uint32_t n,
                                                         uint32_t* free_slots, int32_t* top,
                                                         uint32_t* queue, const uint32_t* tokens,
@@ -800,12 +800,14 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
                                                         uint32_t end_q32, uint64_t* ctr) {
   const uint32_t lane = gallatin::wave::lane();
   uint64_t ended = 0, words = 0, refused = 0, chunks = 0, chunk_fail = 0, freed = 0;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(queue, 64u);
-    base = gallatin::wave::readfirst(base);
-    if (base >= n) break;
-    const uint32_t slot = base + lane;
+  // waves take 64 slots at a time, grid-stride (the launch has about one
+  // wave per 64 slots, so a shared queue counter would only add atomics on
+  // one address)
+  const uint32_t wave_id = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  (void)queue;
+  for (uint64_t base = (uint64_t)wave_id * 64; base < n; base += (uint64_t)nwaves * 64) {
+    const uint32_t slot = (uint32_t)base + lane;
     bool alive = slot < n && ss[slot].alive;
     const uint64_t id = alive ? ss[slot].id : 0;
     const uint64_t r = mix3(seed, id, step);
@@ -824,18 +826,24 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
       free_slots[at0 + (int32_t)gallatin::wave::count(em & gallatin::wave::lanes_below())] = slot;
       ++ended;
     }
+    // each lane reads its own session's size and demand; sessions already
+    // at their 8192-word cap append nothing and stay out of the loop below
+    const bool live = alive && !ends;
+    const uint64_t my_at = live ? ss[slot].vec.size : 0;
+    const uint64_t my_want = live ? (uint64_t)tokens[(r >> 32) & 1023] * 32 : 0;
+    const uint64_t my_room = my_at < 8192 ? 8192 - my_at : 0;
+    const uint64_t my_k = my_want < my_room ? my_want : my_room;
+    if (live && my_k == 0) refused += my_want;
     // appends, one session at a time, all lanes writing
-    uint64_t todo = gallatin::wave::active() & __ballot(alive && !ends);
+    uint64_t todo = gallatin::wave::active() & __ballot(live && my_k > 0);
     for (; todo; todo &= todo - 1) {
       const int l = gallatin::wave::first(todo);
       const uint32_t sl = gallatin::wave::readlane(slot, l);
       const uint64_t sid = gallatin::wave::readlane(id, l);
-      const uint64_t rr = gallatin::wave::readlane(r, l);
       session_vec& v = ss[sl].vec;
-      const uint64_t want = (uint64_t)tokens[(rr >> 32) & 1023] * 32;
-      uint64_t at = gallatin::wave::readfirst(v.size);
-      const uint64_t room = at < 8192 ? 8192 - at : 0;
-      const uint64_t k = want < room ? want : room;
+      const uint64_t want = gallatin::wave::readlane(my_want, l);
+      uint64_t at = gallatin::wave::readlane(my_at, l);
+      const uint64_t k = gallatin::wave::readlane(my_k, l);
       const uint64_t end = at + k;
       while (at < end) {
         const uint32_t d = session_vec::get_directory_index(at);
