@@ -62,7 +62,11 @@ def build_hip(verbose=False, jobs=4):
         if p.wait() != 0:
             raise RuntimeError(f"hipcc failed on {s}")
     tmp = LIB + ".tmp"
-    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs],
+    # export only the C interface (exports.map): the host-side handles of
+    # template kernels instantiated from the public headers stay local, so
+    # a user's instantiations of the same templates are not bound to them
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
+          "-Wl,--version-script=" + os.path.join(HERE, "exports.map"), "-o", tmp, *objs],
          verbose)
     os.replace(tmp, LIB)
     return LIB

@@ -3,15 +3,24 @@
 // into the user's own kernels, its own global allocator symbol, and a
 // Gallatin<> instance it creates and frees itself.  Reference usage followed:
 // tests/src/global_test.cu:33-120 (insert_one_size, free_one_size_pointer)
-// and tests/src/gallatin_test.cu:59-112.
+// and tests/src/gallatin_test.cu:59-112.  user_structures drives the
+// allocator-backed data structures from this unit the same way, after
+// tests/src/vector_tests.cu:121-137, extendible_ht_test.cu:223-249,
+// chaining_test.cu:38-79 and block_queue_test.cu:27-33.
 //
 // Built into tests/dropin/libuser_kernels.so (linked with libgallatin_hip.so)
 // and driven from tests/test_gpu_dropin.py.
 #include <hip/hip_runtime.h>
 
 #include "gallatin/allocators/global_allocator.hpp"
+#include "gallatin/data_structs/block_queue.hpp"
+#include "gallatin/data_structs/chaining_table.hpp"
+#include "gallatin/data_structs/extendible_ht.hpp"
+#include "gallatin/data_structs/fixed_vector.hpp"
+#include "gallatin/data_structs/queue.hpp"
 
 using namespace gallatin::allocators;
+using namespace gallatin::data_structs;
 
 namespace {
 
@@ -88,4 +97,142 @@ extern "C" int user_run(uint64_t pool_bytes, uint64_t n, uint64_t size,
   (void)hipFree(ptrs);
   (void)hipFree(counts);
   return e == hipSuccess ? 0 : -2;
+}
+
+namespace {
+
+using user_vector = fixed_vector<uint64_t, 16, 1ULL << 20>;
+using user_table = extendible_hash_table<uint64_t, 0ULL, ~0ULL, uint64_t, 15, 10, 20>;
+using user_chain = chaining_table<uint64_t, uint64_t, 4>;
+using user_queue = queue<uint64_t, my_allocator>;
+using user_bqueue = block_queue<uint64_t, 256>;
+
+// counts: [0] vector, [1] table, [2] chaining table, [3] queue, [4] block
+// queue (failed operations or wrong values); seen: one bit per item
+
+__global__ void vec_insert(user_vector* v, uint64_t n, uint64_t* pos) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid < n) pos[tid] = v->insert(tid);
+}
+
+__global__ void vec_check(user_vector* v, uint64_t n, const uint64_t* pos, uint64_t* seen,
+                          unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  const uint64_t p = pos[tid];
+  if (p >= n || (*v)[p] != tid) {
+    atomicAdd(&counts[0], 1ULL);
+    return;
+  }
+  const uint64_t bit = 1ULL << (p & 63);
+  if (atomicOr((unsigned long long*)&seen[p >> 6], (unsigned long long)bit) & bit)
+    atomicAdd(&counts[0], 1ULL);  // two items at one position
+}
+
+template <typename T>
+__global__ void map_insert(T* t, uint64_t n, unsigned long long* bad) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid < n && !t->insert(tid + 1, tid)) atomicAdd(bad, 1ULL);
+}
+
+template <typename T>
+__global__ void map_query(T* t, uint64_t n, unsigned long long* bad) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t v = ~0ULL;
+  if (!t->query(tid + 1, v) || v != tid) atomicAdd(bad, 1ULL);
+}
+
+// queue<> reports a failed enqueue through try_enqueue; block_queue's
+// enqueue returns it (block_queue.cuh:331)
+__device__ bool put(user_queue* q, uint64_t x) { return q->try_enqueue(x); }
+__device__ bool put(user_bqueue* q, uint64_t x) { return q->enqueue(x); }
+
+template <typename Q>
+__global__ void q_enqueue(Q* q, uint64_t n, unsigned long long* bad) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid < n && !put(q, tid)) atomicAdd(bad, 1ULL);
+}
+
+template <typename Q>
+__global__ void q_dequeue(Q* q, uint64_t n, uint64_t* seen, unsigned long long* bad) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t v = 0;
+  if (!q->dequeue(v) || v >= n) {
+    atomicAdd(bad, 1ULL);
+    return;
+  }
+  const uint64_t bit = 1ULL << (v & 63);
+  if (atomicOr((unsigned long long*)&seen[v >> 6], (unsigned long long)bit) & bit)
+    atomicAdd(bad, 1ULL);  // an item came out twice
+}
+
+}  // namespace
+
+// Every structure with n items; counts_out[0..4] as above, all 0 when the
+// structures behave.  Needs about 1 GiB of device memory.
+extern "C" int user_structures(uint64_t n, unsigned long long* counts_out) {
+  init_global_allocator(1ULL << 30, 11, false);
+  unsigned long long* counts = nullptr;
+  uint64_t *pos = nullptr, *seen = nullptr;
+  const uint64_t words = (n + 63) / 64;
+  if (hipMalloc(&counts, 5 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&pos, n * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&seen, words * sizeof(uint64_t)) != hipSuccess)
+    return -1;
+  (void)hipMemset(counts, 0, 5 * sizeof(unsigned long long));
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+
+  user_vector* v = user_vector::get_device_vector();
+  if (!v) return -2;
+  (void)hipMemset(seen, 0, words * sizeof(uint64_t));
+  vec_insert<<<blocks, 256>>>(v, n, pos);
+  vec_check<<<blocks, 256>>>(v, n, pos, seen, counts);
+  if (hipDeviceSynchronize() != hipSuccess) return -3;
+  user_vector::free_device_vector(v);
+
+  user_table* t = user_table::generate_on_device(42);
+  if (!t) return -4;
+  map_insert<<<blocks, 256>>>(t, n, counts + 1);
+  map_query<<<blocks, 256>>>(t, n, counts + 1);
+  if (hipDeviceSynchronize() != hipSuccess) return -5;
+  user_table::free_on_device(t);
+
+  user_chain* c = user_chain::generate_on_device(n, 0, 123456ULL);
+  if (!c) return -6;
+  map_insert<<<blocks, 256>>>(c, n, counts + 2);
+  map_query<<<blocks, 256>>>(c, n, counts + 2);
+  if (hipDeviceSynchronize() != hipSuccess) return -7;
+  c->print_chain_stats();
+  user_chain::free_on_device(c);
+  free_global_allocator();
+
+  my_allocator* alloc = my_allocator::generate_on_device(1ULL << 30, 5, false);
+  if (!alloc) return -8;
+  user_queue* q = user_queue::generate_on_device(alloc);
+  if (!q) return -9;
+  (void)hipMemset(seen, 0, words * sizeof(uint64_t));
+  q_enqueue<<<blocks, 256>>>(q, n, counts + 3);
+  q_dequeue<<<blocks, 256>>>(q, n, seen, counts + 3);
+  if (hipDeviceSynchronize() != hipSuccess) return -10;
+  user_queue::free_on_device(q);
+  my_allocator::free_on_device(alloc);
+
+  init_global_allocator(1ULL << 30, 12, false);
+  user_bqueue* b = user_bqueue::generate_on_device();
+  if (!b) return -11;
+  (void)hipMemset(seen, 0, words * sizeof(uint64_t));
+  q_enqueue<<<blocks, 256>>>(b, n, counts + 4);
+  q_dequeue<<<blocks, 256>>>(b, n, seen, counts + 4);
+  if (hipDeviceSynchronize() != hipSuccess) return -12;
+  user_bqueue::free_on_device(b);
+  free_global_allocator();
+
+  const hipError_t e = hipMemcpy(counts_out, counts, 5 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost);
+  (void)hipFree(counts);
+  (void)hipFree(pos);
+  (void)hipFree(seen);
+  return e == hipSuccess ? 0 : -13;
 }

@@ -18,11 +18,13 @@ def _lib():
     L.user_run.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                            ctypes.POINTER(ctypes.c_ulonglong)]
     L.user_run.restype = ctypes.c_int
+    L.user_structures.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.user_structures.restype = ctypes.c_int
     return L
 
 
 def test_user_library_loads_and_exports():
-    assert hasattr(_lib(), "user_run")
+    assert hasattr(_lib(), "user_run") and hasattr(_lib(), "user_structures")
 
 
 @pytest.mark.gpu
@@ -36,3 +38,17 @@ def test_user_kernels_global_and_own_instance(size, n):
     rc = _lib().user_run(1 << 30, n, size, counts)
     assert rc == 0
     assert list(counts) == [0, 0, 0, 0], "misses / errors in the user kernels"
+
+
+@pytest.mark.gpu
+def test_user_kernels_data_structures():
+    """The user unit includes the data-structure headers and runs each the
+    way the reference's tests do: a vector's items at unique positions, hash
+    and chaining tables queried back after inserts (keys tid + 1, values
+    tid), and two queues that hand back every item exactly once."""
+    import torch
+    torch.cuda.set_device(0)
+    counts = (ctypes.c_ulonglong * 5)()
+    rc = _lib().user_structures(1 << 16, counts)
+    assert rc == 0
+    assert list(counts) == [0] * 5, "vector, table, chain, queue, block queue errors"
