@@ -16,6 +16,10 @@ peak, and its atomic issue rate against the measured atomic ceiling.
 on one host core; multithreaded glibc malloc/free of the same streams is
 reported beside it.
 
+Beside the headline: the plain small method (1 GiB, malloc(16)), the block
+and multi-segment batches (64 GiB), the batched Zipf stream with its
+autotuner sweep and the serving loop, each with its own workload string.
+
   python bench.py [--steps K] [--warmup W] [--path batched|plain|both]
 """
 import argparse
@@ -57,6 +61,9 @@ def parse():
                    help="skip the batched Zipf-stream sweep")
     p.add_argument("--zipf-pool-gib", type=int, default=64)
     p.add_argument("--zipf-reps", type=int, default=3)
+    p.add_argument("--no-big", action="store_true",
+                   help="skip the block and multi-segment batches")
+    p.add_argument("--big-steps", type=int, default=20)
     p.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     p.add_argument("--serving-pool-gib", type=int, default=200)
     p.add_argument("--serving-steps", type=int, default=200)
@@ -195,6 +202,48 @@ def plain_small(args, G, torch, flags=0):
             "violations": c[G.CTR_BAD] + info["errors"]}
 
 
+def block_multiseg(args, G, torch):
+    """Block and multi-segment path: 64 GiB pool (4096 segments, three-level
+    trees).  One step is one batch of 4096 requests uniform in [64 KiB,
+    64 MiB] and 64 uniform in [256 MiB, 2 GiB]: a launch that mallocs and
+    tags each one, then a launch that checks and frees them in a
+    pseudo-random order.  Demand is far above the pool, so many requests
+    miss.  After the timed steps a drain must leave every segment free."""
+    a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(64 << 30, args.seed,
+                                                          print_info=False)
+    ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
+    ptrs = torch.zeros(4096 + 64, dtype=torch.int64, device="cuda")
+    warm, steps = 3, args.big_steps
+    for s in range(warm):
+        a.run_big(args.seed, s, ptrs=ptrs, counters=ctr)
+    torch.cuda.synchronize()
+    ctr.zero_()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record()
+    for i, s in enumerate(range(warm, warm + steps)):
+        a.run_big(args.seed, s, ptrs=ptrs, counters=ctr)
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    secs = ev[0].elapsed_time(ev[-1]) / 1e3
+    a.drain()
+    info = a.info()
+    a.free_on_device()
+    return {"workload": "64 GiB pool, per step 4096 requests uniform in [64 KiB, 64 MiB] + 64 "
+                        "in [256 MiB, 2 GiB]: malloc and tag, then check and free in a "
+                        "pseudo-random order (two launches)",
+            "value": c[G.CTR_PAIRS] / secs, "unit": "pairs/s", "steps": steps,
+            "ms_per_step": secs * 1e3 / steps, "p50_step_ms": per[steps // 2],
+            "p99_step_ms": per[min(steps - 1, (99 * steps) // 100)],
+            "requests_per_step": 4160, "pairs_per_step": c[G.CTR_PAIRS] / steps,
+            "misses_per_step": c[G.CTR_MISSES] / steps,
+            "violations": c[G.CTR_BAD] + info["errors"],
+            "fresh_after_drain": info["segments_free"] == info["num_segments"]
+                                 and info["segments_formatted"] == 0
+                                 and info["segments_multi"] == 0}
+
+
 def zipf_sweep(args, G, torch):
     """Batched persistent form: 256 batches with Zipf(1.2) request counts on
     [1, 2^20], 60 % malloc of lognormal(6, 2) bytes clipped to [1, 16 MiB] /
@@ -290,6 +339,18 @@ def cpu_baselines(args):
                         "kind": "glibc malloc/free",
                         "sample": f"all {args.threads} stream threads, {r.pairs} pairs "
                                   f"in {r.seconds:.2f} s, pattern errors {r.bad}"}
+    if not args.no_big:
+        lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                          ctypes.POINTER(R)]
+        r = R()
+        lib.gal_cpu_big_glibc(args.seed, 3, args.big_steps, 4096, 64, cores, ctypes.byref(r))
+        res["cpu_glibc_block_multiseg"] = {
+            "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+            "kind": "glibc malloc/free",
+            "sample": f"the same {args.big_steps} block and multi-segment steps, {r.pairs} "
+                      f"pairs (glibc serves every request) in {r.seconds:.2f} s, tag errors "
+                      f"{r.bad}"}
     return res
 
 
@@ -426,6 +487,8 @@ def main():
     # pool too small for XCD partitions)
     out["plain_small"]["coherent_payload"] = plain_small(
         args, G, torch, flags=G.GAL_FLAG_NO_PAYLOAD_FENCE)
+    if not args.no_big:
+        out["block_multiseg"] = block_multiseg(args, G, torch)
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)
     if not args.no_serving:

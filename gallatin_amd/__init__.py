@@ -44,7 +44,8 @@ CAPI_SYMBOLS = [
     "gal_free_global", "gal_print_global_stats", "gal_global_handle",
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
-    "gal_run_churn", "gal_run_churn_batched", "gal_bench_atomic", "gal_bench_copy",
+    "gal_run_churn", "gal_run_churn_batched", "gal_run_big", "gal_bench_atomic",
+    "gal_bench_copy",
     "gal_bench_scatter", "gal_bench_tree",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
     "gal_batch_plan_launch", "gal_batch_plan_destroy",
@@ -129,6 +130,7 @@ def lib():
         L.gal_run_plain.argtypes = [_p, _u64, _u64, _p, _u32, _p]
         L.gal_run_churn.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _p]
         L.gal_run_churn_batched.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _p]
+        L.gal_run_big.argtypes = [_p, _u64, _u32, _u64, _u64, _p, _p, _p]
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
         L.gal_bench_scatter.argtypes = [_p, _p, _u64, _u64, _p]
@@ -367,6 +369,22 @@ class Gallatin:
         _check(lib().gal_run_churn(self.handle, seed, nthreads, rounds,
                                    ctr.data_ptr(), block_threads, _stream()))
         return ctr
+
+    def run_big(self, seed, step, n_small=4096, n_large=64, ptrs=None, counters=None):
+        """One step of the block and multi-segment batch: n_small requests in
+        [64 KiB, 64 MiB] and n_large in [256 MiB, 2 GiB] malloc and tag, then
+        are checked and freed in a pseudo-random order.  Returns (ptrs,
+        counters); ptrs keeps what the malloc launch returned."""
+        import torch
+        n = n_small + n_large
+        if ptrs is None:
+            ptrs = torch.zeros(n, dtype=torch.int64, device="cuda")
+        if ptrs.numel() < n or ptrs.dtype != torch.int64 or not ptrs.is_cuda:
+            raise ValueError("ptrs must be a device int64 tensor of n_small + n_large entries")
+        ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+        _check(lib().gal_run_big(self.handle, seed, step, n_small, n_large, ptrs.data_ptr(),
+                                 ctr.data_ptr(), _stream()))
+        return ptrs, ctr
 
     def run_churn_batched(self, seed, nthreads, rounds, counters=None, workgroups=0,
                           threads=0):

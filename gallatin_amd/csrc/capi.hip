@@ -83,7 +83,7 @@ __global__ void k_boot(gal_state* s) {
   for (uint32_t t = 0; t < s->num_trees; ++t) {
     const uint32_t per = s->slot_count[t] / s->num_parts;
     for (uint32_t i = 0; i < s->slot_count[t]; ++i) {
-      const uint32_t blk = detail::request_new_block(s, t, i / per, true);
+      const uint32_t blk = detail::request_new_block(s, t, i / per, true, s->lim_request);
       if (blk == kNullBlock) {
         if (wave::is_first()) utils::atomic_store(&s->stats[kStatBootFailed], 1ULL);
         return;

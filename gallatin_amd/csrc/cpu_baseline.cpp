@@ -1,7 +1,8 @@
-// Host baseline: the mixed-size churn request stream of streams.h run through
-// glibc malloc/free on the host cores, one std::thread per core, each thread
-// taking a contiguous shard of request-stream thread ids and doing the same
-// fill / verify / free work as the GPU churn kernel.
+// Host baselines: request streams of streams.h run through glibc malloc/free
+// on the host cores, one std::thread per core.  The mixed-size churn gives
+// each thread a contiguous shard of request-stream thread ids and does the
+// same fill / verify / free work as the GPU churn kernel; the block and
+// multi-segment batches do the same tag / check / free work as the GPU's.
 #include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -74,6 +75,75 @@ int gal_cpu_churn_glibc(uint64_t seed, uint64_t tid0, uint64_t n, uint32_t round
     const uint64_t lo = tid0 + n * i / threads, hi = tid0 + n * (i + 1) / threads;
     pool.emplace_back(work, lo, hi);
   }
+  for (auto& t : pool) t.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  out->pairs = pairs;
+  out->misses = misses;
+  out->bad = bad;
+  out->bytes = bytes;
+  out->seconds = std::chrono::duration<double>(t1 - t0).count();
+  out->threads = threads;
+  out->pad = 0;
+  return 0;
+}
+
+// Block and multi-segment batches of streams.h (gal_big_*), steps step0 ..
+// step0 + steps - 1, on `threads` host threads that stay up for the whole
+// run.  Each step: every thread mallocs a contiguous shard of the requests
+// and tags each one's first word; after a barrier every thread checks and
+// frees a contiguous shard of the free order; a second barrier ends the step.
+int gal_cpu_big_glibc(uint64_t seed, uint32_t step0, uint32_t steps, uint64_t n_small,
+                      uint64_t n_large, uint32_t threads, gal_cpu_result* out) {
+  if (!out) return -1;
+  if (threads == 0) threads = (uint32_t)gal_cpu_online_cores();
+  const uint64_t n = n_small + n_large;
+  std::vector<uint64_t*> ptrs(n, nullptr);
+  std::atomic<uint64_t> pairs{0}, misses{0}, bad{0}, bytes{0};
+  std::atomic<uint32_t> arrived{0}, phase{0};
+  auto barrier = [&](uint32_t& local) {
+    local ^= 1;
+    if (arrived.fetch_add(1) + 1 == threads) {
+      arrived.store(0);
+      phase.store(local);
+    } else {
+      while (phase.load() != local) std::this_thread::yield();
+    }
+  };
+  auto work = [&](uint32_t me) {
+    uint64_t p = 0, m = 0, b = 0, by = 0;
+    uint32_t local = 0;
+    const uint64_t lo = n * me / threads, hi = n * (me + 1) / threads;
+    for (uint32_t s = step0; s < step0 + steps; ++s) {
+      for (uint64_t i = lo; i < hi; ++i) {
+        uint64_t* q = static_cast<uint64_t*>(malloc(gal_big_size(seed, s, i, n_small)));
+        if (q) {
+          q[0] = gal_big_tag(s, i);
+          by += 8;
+        } else {
+          ++m;
+        }
+        ptrs[i] = q;
+      }
+      barrier(local);
+      for (uint64_t j = lo; j < hi; ++j) {
+        const uint64_t i = gal_big_free_order(seed, s, j, n);
+        uint64_t* q = ptrs[i];
+        if (!q) continue;
+        b += q[0] != gal_big_tag(s, i);
+        by += 8;
+        free(q);
+        ++p;
+      }
+      barrier(local);
+    }
+    pairs += p;
+    misses += m;
+    bad += b;
+    bytes += by;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  for (uint32_t i = 0; i < threads; ++i) pool.emplace_back(work, i);
   for (auto& t : pool) t.join();
   const auto t1 = std::chrono::steady_clock::now();
   out->pairs = pairs;

@@ -252,6 +252,45 @@ __global__ void k_churn(gal_state* s, uint64_t seed, uint64_t n, uint32_t rounds
   add_counters(ctr, c);
 }
 
+// Block and multi-segment batch, allocation half: request i of the step
+// mallocs its size and tags its first word.
+__global__ void k_big_malloc(gal_state* s, uint64_t seed, uint32_t step, uint64_t n,
+                             uint64_t n_small, void** ptrs, uint64_t* ctr) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  lane_counts c;
+  if (i < n) {
+    uint64_t* p = static_cast<uint64_t*>(
+        detail::malloc_impl(s, gal_big_size(seed, step, i, n_small)));
+    c.mallocs = 1;
+    if (p) {
+      p[0] = gal_big_tag(step, i);
+      c.bytes = 8;
+    } else {
+      c.misses = 1;
+    }
+    ptrs[i] = p;
+  }
+  add_counters(ctr, c);
+}
+
+// Free half: the j-th thread checks and frees request gal_big_free_order(j).
+__global__ void k_big_free(gal_state* s, uint64_t seed, uint32_t step, uint64_t n,
+                           void* const* ptrs, uint64_t* ctr) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  lane_counts c;
+  if (j < n) {
+    const uint64_t i = gal_big_free_order(seed, step, j, n);
+    uint64_t* p = static_cast<uint64_t*>(ptrs[i]);
+    if (p) {
+      c.bad = p[0] != gal_big_tag(step, i);
+      c.bytes = 8;
+      detail::free_impl(s, p);
+      c.pairs = 1;
+    }
+  }
+  add_counters(ctr, c);
+}
+
 // The same churn through the workgroup-batched path: a persistent grid walks
 // chunks of blockDim stream threads; for every round the workgroup frees its
 // replaced allocations and allocates the new ones as one batch step.
@@ -469,6 +508,20 @@ int gal_run_churn(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,
   if (threads == 0 || threads > 1024 || threads % 64) threads = 256;
   k_churn<<<grid_for(n, threads), threads, 0, (hipStream_t)stream>>>(a->dev, seed, n, rounds, ctr);
   return launch_status("churn");
+}
+
+int gal_run_big(gal_allocator* a, uint64_t seed, uint32_t step, uint64_t n_small,
+                uint64_t n_large, void** ptrs, uint64_t* ctr, void* stream) {
+  if (!a || !ctr || !ptrs) return fail(GAL_ERR_INVALID, "null argument");
+  const uint64_t n = n_small + n_large;
+  if (n == 0) return GAL_OK;
+  if (n > (1ULL << 32)) return fail(GAL_ERR_INVALID, "at most 2^32 requests per step");
+  hipStream_t st = (hipStream_t)stream;
+  k_big_malloc<<<grid_for(n, 256), 256, 0, st>>>(a->dev, seed, step, n, n_small, ptrs, ctr);
+  int rc = launch_status("big_malloc");
+  if (rc != GAL_OK) return rc;
+  k_big_free<<<grid_for(n, 256), 256, 0, st>>>(a->dev, seed, step, n, ptrs, ctr);
+  return launch_status("big_free");
 }
 
 int gal_run_churn_batched(gal_allocator* a, uint64_t seed, uint64_t n, uint32_t rounds,

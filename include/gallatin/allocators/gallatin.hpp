@@ -360,16 +360,24 @@ __device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bo
 }
 
 // gallatin.cuh:1525-1631 request_new_block_from_tree.  With no segment in
-// the class tree and none free it polls (up to lim_request times): frees in
-// flight on other waves return segments within microseconds.
+// the class tree and none free it polls (up to `polls` times, lim_request
+// for a first request): frees in flight on other waves return segments
+// within microseconds.
 __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t p,
-                                             bool det) {
+                                             bool det, uint32_t polls) {
   int attempts = 0;
   const uint32_t lk = tree_lock(s, t, p);
-  for (uint32_t spins = 0; attempts < (int)s->lim_request && spins < kSpinCap;
+  for (uint32_t spins = 0; attempts < (int)polls && spins < kSpinCap;
        ++spins) {
     uint64_t seg = find_segment(s, t, p, det);
     if (seg == kFail) {
+      if (veb::looks_empty(s->seg_tree)) {
+        // nothing free: poll without taking the class lock, whose release
+        // writes back the L2 and which every waiting wave would queue on
+        ++attempts;
+        utils::backoff();
+        continue;
+      }
       if (!try_lock(s, lk)) {
         utils::backoff();
         continue;
@@ -412,7 +420,7 @@ __device__ __noinline__ uint32_t refill_slot(gal_state* s, uint32_t t, uint32_t 
   if (wave::is_first()) cur = atomic_cas(sp, kNullBlock, kRefilling);
   cur = wave::readfirst(cur);
   if (cur != kNullBlock) return cur;  // a block, or another wave's refill
-  const uint32_t nb = request_new_block(s, t, p, det);
+  const uint32_t nb = request_new_block(s, t, p, det, s->lim_request);
   if (wave::is_first()) {
     atomic_exch(sp, nb);  // the block, or empty again
     if (nb != kNullBlock) add_stat(s, kStatRefills, 1);
@@ -437,7 +445,7 @@ __device__ __noinline__ void replace_block(gal_state* s, uint32_t t, uint32_t p,
       retire_block(s, blk, p);
   }
   if (wave::readfirst(seen) != blk) return;
-  const uint32_t nb = request_new_block(s, t, p, det);
+  const uint32_t nb = request_new_block(s, t, p, det, s->lim_request);
   if (wave::is_first()) atomic_exch(sp, nb);
 }
 
@@ -549,9 +557,9 @@ __device__ __forceinline__ void malloc_slices(gal_state* s, bool want, uint32_t 
 // malloc_fill_block, retried as gallatin.cuh:1226-1231 while blocks are
 // handed out but lost to a concurrent claim.  Byte offset.
 __device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, uint32_t p,
-                                                   bool det) {
+                                                   bool det, uint32_t polls) {
   for (uint32_t attempt = 0; attempt < s->lim_block; ++attempt) {
-    const uint32_t blk = request_new_block(s, bt, p, det);
+    const uint32_t blk = request_new_block(s, bt, p, det, polls);
     if (blk == kNullBlock) return kFail;
     uint32_t res = 0;
     if (wave::is_first()) {
@@ -572,6 +580,33 @@ __device__ __noinline__ uint64_t malloc_block_path(gal_state* s, uint32_t bt, ui
   return kFail;
 }
 
+// set_tree_id (memory_table.cuh:823-828) for the head of an n-segment run:
+// chunk id T + 1 + n (per lane).  On failure the run goes back to the
+// segment tree and the error is counted.
+__device__ inline bool mark_multi_head(gal_state* s, uint64_t idx, uint64_t n) {
+  const uint32_t T = s->num_trees;
+  uint64_t cur = atomic_load(&s->seg_state[idx]);
+  for (int g = 0; g < 4096; ++g) {
+    if ((cur & 0xFFFF) != kFreeTree) break;
+    const uint64_t seen = atomic_cas(&s->seg_state[idx], cur,
+                                     (uint64_t)((cur & ~kSegIdMask) | (T + 1 + n)));
+    if (seen == cur) return true;
+    cur = seen;
+  }
+  add_error(s, kStatErrMultiHead);
+  veb::return_multiple(s->seg_tree, idx, n);
+  return false;
+}
+
+// Take the multi-segment lock (lock 0, gallatin.cuh:796).
+__device__ inline bool lock_segments(gal_state* s) {
+  for (uint32_t spin = 0; spin < kSpinCap; ++spin) {
+    if (try_lock(s, 0)) return true;
+    utils::backoff();
+  }
+  return false;
+}
+
 // gallatin.cuh:790-826 malloc_segment_allocation under the segment lock,
 // retried as gallatin.cuh:1246-1251.  Byte offset.  The lock is taken by one
 // lane of one request at a time, so lanes of a wave never wait on each other.
@@ -579,41 +614,57 @@ __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
   const uint32_t T = s->num_trees;
   if (n == 0 || n > s->num_segments || n + T + 1 >= kFreeTree) return kFail;
   for (uint32_t attempt = 0; attempt < s->lim_segment; ++attempt) {
-    bool got = false;
-    for (uint32_t spin = 0; spin < kSpinCap && !got; ++spin) {
-      got = try_lock(s, 0);
-      if (!got) utils::backoff();
-    }
-    if (!got) return kFail;
+    if (!lock_segments(s)) return kFail;
     const uint64_t idx = veb::gather_multiple(s->seg_tree, n);
     if (idx == kFail) {  // no free run of n segments now
       unlock(s, 0);
       return kFail;
     }
     uint32_t ok = 0;
-    if (wave::is_first()) {
-      // set_tree_id (memory_table.cuh:823-828): head marked T + 1 + n
-      uint64_t cur = atomic_load(&s->seg_state[idx]);
-      for (int g = 0; g < 4096; ++g) {
-        if ((cur & 0xFFFF) != kFreeTree) break;
-        const uint64_t seen = atomic_cas(&s->seg_state[idx], cur,
-                                         (uint64_t)((cur & ~kSegIdMask) | (T + 1 + n)));
-        if (seen == cur) {
-          ok = 1;
-          break;
-        }
-        cur = seen;
-      }
-      if (!ok) {
-        add_error(s, kStatErrMultiHead);
-        veb::return_multiple(s->seg_tree, idx, n);
-      }
-    }
+    if (wave::is_first()) ok = mark_multi_head(s, idx, n);
     ok = wave::readfirst(ok);
     unlock(s, 0);
     if (ok) return idx << s->seg_shift;
   }
   return kFail;
+}
+
+// The multi-segment requests of a wave (lanes with `want`, n segments each).
+// Several of them claim one run of their total length under one taking of
+// the lock and split it in lane order, each lane marking the head of its own
+// part, so a wave serialises one claim instead of one per lane; every part
+// is an ordinary n-segment allocation (freed on its own).  Without a run of
+// the total (or in the deterministic mode, whose replay takes runs one lane
+// at a time as the reference does) the lanes claim one after another, and a
+// size that just found no run fails the wave's later requests of that size
+// or more without another scan.  Byte offset per lane, kFail on a miss.
+__device__ __noinline__ uint64_t malloc_segments(gal_state* s, bool want, uint64_t n,
+                                                 bool det) {
+  const uint32_t T = s->num_trees;
+  const uint64_t wm = __ballot(want);
+  const bool ok_n = n > 0 && n <= s->num_segments && n + T + 1 < kFreeTree;
+  uint64_t out = kFail;
+  if (!det && wave::count(wm) > 1 && !__ballot(want && !ok_n)) {
+    uint32_t pre = 0, total = 0;
+    wave::group_scan(want ? (uint32_t)n : 0u, wm, pre, total);
+    if (total <= s->num_segments && lock_segments(s)) {
+      const uint64_t idx = veb::gather_multiple(s->seg_tree, total);
+      unlock(s, 0);
+      if (idx != kFail) {
+        if (want && mark_multi_head(s, idx + pre, n)) out = (idx + pre) << s->seg_shift;
+        return out;
+      }
+    }
+  }
+  uint64_t no_run = ~0ULL;  // smallest size that found no run
+  for (uint64_t m = wm; m; m &= m - 1) {
+    const int l = wave::first(m);
+    const uint64_t nl = wave::readlane(n, l);
+    const uint64_t r = nl >= no_run ? kFail : malloc_segment_path(s, nl);
+    if (r == kFail && nl < no_run) no_run = nl;
+    if ((int)wave::lane() == l) out = r;
+  }
+  return out;
 }
 
 // gallatin.cuh:1186-1276 malloc.
@@ -647,15 +698,20 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
     malloc_slices(s, path == kPathSlice, t, cnt, p, off);
     if (off != kFail) bytes = offset_to_bytes(s, off, t);
   }
+  // The block requests of a wave run one after another.  Once one of them
+  // has polled out its attempts with no segment to be had, the rest try
+  // once each (any class tree may still hold a segment with a free block)
+  // instead of polling the same empty pool again.
+  uint32_t polls = s->lim_request;
   for (uint64_t m = __ballot(path == kPathBlock); m; m &= m - 1) {
     const int l = wave::first(m);
-    const uint64_t r = malloc_block_path(s, wave::readlane(t, l), p, det);
+    const uint64_t r = malloc_block_path(s, wave::readlane(t, l), p, det, polls);
+    if (r == kFail) polls = 1;
     if ((int)lane == l) bytes = r;
   }
-  for (uint64_t m = __ballot(path == kPathSegment); m; m &= m - 1) {
-    const int l = wave::first(m);
-    const uint64_t r = malloc_segment_path(s, wave::readlane(nseg, l));
-    if ((int)lane == l) bytes = r;
+  if (__ballot(path == kPathSegment)) {
+    const uint64_t r = malloc_segments(s, path == kPathSegment, nseg, det);
+    if (path == kPathSegment) bytes = r;
   }
   // With one partition a slice may come from any XCD's past: acquire at
   // agent scope.  With XCD partitions the previous users of a slice wrote

@@ -129,6 +129,22 @@ __device__ inline uint64_t find_ge(const gal_tree& t, uint64_t x) {
   return kFail;
 }
 
+// True when the top level shows no available index (no atomics taken).
+// While other waves update the tree the answer can be stale by one update,
+// so callers use it only to skip work they would repeat anyway.
+__device__ inline bool looks_empty(const gal_tree& t) {
+  const uint64_t act = wave::active();
+  const uint32_t k = wave::count(act);
+  const uint32_t rank = wave::count(act & wave::lanes_below());
+  const uint64_t nwords = t.layer_words[0];
+  for (uint64_t w0 = 0; w0 < nwords; w0 += k) {
+    const uint64_t w = w0 + rank;
+    const uint64_t v = w < nwords ? atomic_load(word(t, 0, w)) : 0;
+    if (__ballot(v != 0)) return false;
+  }
+  return true;
+}
+
 // malloc_first (veb.cuh:979-991): take the smallest available index.
 __device__ inline uint64_t take_first(const gal_tree& t) {
   for (int attempt = 0; attempt < 15; ++attempt) {

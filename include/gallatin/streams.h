@@ -101,6 +101,52 @@ static inline GAL_HD uint64_t gal_fill_words(uint64_t size) {
   return (size + 7) >> 3;
 }
 
+/* ---- block and multi-segment requests (one batch per step) ----
+ * Requests 0 .. n_small-1 are uniform in [64 KiB, 64 MiB] (whole blocks of
+ * the block classes, or 2-4 segment runs at 16 MiB segments); the rest are
+ * uniform in [256 MiB, 2 GiB] (16-128 segment runs).  Every served request
+ * gets a u64 tag in its first word, checked before it is freed, and the
+ * frees of a step run in a pseudo-random order of the requests. */
+
+#define GAL_BIG_USE_SIZE 3u
+#define GAL_BIG_USE_ORDER 4u
+
+static inline GAL_HD uint64_t gal_big_size(uint64_t seed, uint32_t step, uint64_t i,
+                                           uint64_t n_small) {
+  const uint64_t h = gal_hash4(seed, i, step, GAL_BIG_USE_SIZE);
+  if (i < n_small) return (64ULL << 10) + h % ((64ULL << 20) - (64ULL << 10) + 1);
+  return (256ULL << 20) + h % ((2ULL << 30) - (256ULL << 20) + 1);
+}
+
+static inline GAL_HD uint64_t gal_big_tag(uint32_t step, uint64_t i) {
+  return gal_mix64(((uint64_t)step << 40) ^ i);
+}
+
+/* Request freed by the j-th free of a step: a bijection of [0, n), made by
+ * walking the cycles of a bijection of [0, 2^k) (2^k >= n) until the value
+ * falls below n.  The bijection is three rounds of xor with a constant,
+ * multiplication by an odd constant mod 2^k and x ^= x >> ceil(k/2), each
+ * invertible on k bits.  The walk ends: the cycle through j < n returns to
+ * j itself at the latest. */
+static inline GAL_HD uint64_t gal_big_free_order(uint64_t seed, uint32_t step, uint64_t j,
+                                                 uint64_t n) {
+  uint32_t k = 1;
+  while ((1ULL << k) < n) ++k;
+  const uint64_t mask = (1ULL << k) - 1;
+  const uint64_t h = gal_hash4(seed, step, 0, GAL_BIG_USE_ORDER);
+  const uint64_t g = gal_mix64(h);
+  uint64_t x = j;
+  do {
+    for (uint32_t r = 0; r < 3; ++r) {
+      const uint64_t c = (h >> (20 * r)) & mask;
+      const uint64_t m = ((g >> (20 * r)) | 1) & mask;
+      x = ((x ^ c) * m) & mask;
+      x ^= x >> ((k + 1) / 2);
+    }
+  } while (x >= n);
+  return x;
+}
+
 #ifdef __cplusplus
 }
 #endif

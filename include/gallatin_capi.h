@@ -206,6 +206,16 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
                   uint32_t rounds, uint64_t *counters_dev,
                   uint32_t block_threads, void *stream);
 
+/* Block and multi-segment batch, one step of streams.h's gal_big_* stream:
+ * a launch in which n_small requests uniform in [64 KiB, 64 MiB] and n_large
+ * uniform in [256 MiB, 2 GiB] each malloc and tag their first word, then a
+ * launch that checks and frees them in gal_big_free_order.  ptrs_dev holds
+ * n_small + n_large pointers (written by the first launch).  The request
+ * shapes exercise malloc_block_allocation and malloc_segment_allocation
+ * (gallatin.cuh:790-861) and the multi-segment free (gallatin.cuh:1378-1461). */
+int gal_run_big(gal_allocator *a, uint64_t seed, uint32_t step, uint64_t n_small,
+                uint64_t n_large, void **ptrs_dev, uint64_t *counters_dev, void *stream);
+
 /* The same churn through the workgroup-batched path (batched.hpp): one
  * persistent launch of `workgroups` workgroups (0 = fill the device) of
  * `threads` threads (256, 512 or 1024; 0 = 1024) walks chunks of `threads`

@@ -52,6 +52,8 @@ def lib():
         L.orc_murmur64a.argtypes = [_p, ctypes.c_int, _u64]
         L.orc_chain_of.restype = None
         L.orc_chain_of.argtypes = [_p, ctypes.c_int, _u64, _u64, _u64, _p]
+        L.orc_big_stream.restype = None
+        L.orc_big_stream.argtypes = [_u64, ctypes.c_uint32, _u64, _u64, _p, _p]
         L.orc_veb_create.restype = _p
         L.orc_veb_create.argtypes = [_u64, ctypes.c_int]
         L.orc_veb_destroy.argtypes = [_p]
@@ -112,6 +114,17 @@ def chain_of(keys, seed: int, nchains: int):
     out = np.zeros(len(k), dtype=np.uint64)
     lib().orc_chain_of(k.ctypes.data, k.dtype.itemsize, len(k), seed, nchains, out.ctypes.data)
     return out
+
+
+def big_stream(seed: int, step: int, n_small: int, n_large: int):
+    """(sizes, free order) of one step of the block and multi-segment
+    request stream, from the header the GPU kernels use."""
+    import numpy as np
+    n = n_small + n_large
+    sizes = np.zeros(n, dtype=np.uint64)
+    order = np.zeros(n, dtype=np.uint64)
+    lib().orc_big_stream(seed, step, n_small, n_large, sizes.ctypes.data, order.ctypes.data)
+    return sizes, order
 
 
 def clog2(x: int) -> int:

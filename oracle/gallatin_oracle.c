@@ -48,6 +48,19 @@ void orc_chain_of(const void *keys, int key_bytes, uint64_t n, uint64_t seed,
     out[i] = orc_murmur64a(p + i * (uint64_t)key_bytes, key_bytes, seed) % nchains;
 }
 
+/* The block and multi-segment request stream of streams.h (no reference
+ * counterpart: the request shapes of the block + multi-segment workload),
+ * evaluated by the same header the GPU kernels include: sizes[i] and the
+ * request order[j] freed j-th in step `step`. */
+void orc_big_stream(uint64_t seed, uint32_t step, uint64_t n_small, uint64_t n_large,
+                    uint64_t *sizes, uint64_t *order) {
+  const uint64_t n = n_small + n_large;
+  for (uint64_t i = 0; i < n; i++) {
+    if (sizes) sizes[i] = gal_big_size(seed, step, i, n_small);
+    if (order) order[i] = gal_big_free_order(seed, step, i, n);
+  }
+}
+
 /* alloc_utils.cuh:440-453 __cfcll: number of trailing one bits. */
 int orc_cfcll(uint64_t bits) {
   int p = __builtin_popcountll(bits);

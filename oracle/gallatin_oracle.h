@@ -63,6 +63,8 @@ int orc_cfcll(uint64_t bits);
 uint64_t orc_murmur64a(const void *key, int len, uint64_t seed);
 void orc_chain_of(const void *keys, int key_bytes, uint64_t n, uint64_t seed,
                   uint64_t nchains, uint64_t *out);
+void orc_big_stream(uint64_t seed, uint32_t step, uint64_t n_small, uint64_t n_large,
+                    uint64_t *sizes, uint64_t *order);
 
 /* ------------------------------------------------------------- vEB tree */
 typedef struct orc_veb {

@@ -240,6 +240,80 @@ def test_block_and_multi_segment_config(gpu):
     a.free_on_device()
 
 
+def test_block_and_multi_segment_steps(gpu):
+    """Three steps of the block and multi-segment batch (gal_run_big, the
+    workload bench.py times).  In each step the served requests were live
+    together, so they must be in the pool, aligned and disjoint; served plus
+    misses is every request; every tag checks; every served request is
+    freed.  After the steps a drain leaves the fresh state bit for bit."""
+    import torch
+    seed, n_small, n_large = 77, 4096, 64
+    a = make(gpu, 64 << 30, det=False, limits=None)
+    ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
+    for step in range(3):
+        ctr.zero_()
+        ptrs, _ = a.run_big(seed, step, n_small, n_large, counters=ctr)
+        torch.cuda.synchronize()
+        p = ptrs.cpu().numpy().view(np.uint64)
+        sizes, _ = oracle.big_stream(seed, step, n_small, n_large)
+        served = int((p != 0).sum())
+        c = ctr.cpu().tolist()
+        assert served > 0
+        assert c[gpu.CTR_MALLOCS] == n_small + n_large
+        assert c[gpu.CTR_PAIRS] == served and c[gpu.CTR_MISSES] == n_small + n_large - served
+        assert c[gpu.CTR_BAD] == 0
+        assert c[gpu.CTR_BYTES] == 16 * served
+        H.check_ptrs(p, sizes, a.pool_base, a.num_segments * S)
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
+def test_wave_of_multi_segment_requests(gpu):
+    """Lanes of one wave asking for multi-segment runs.  When their total
+    fits, the wave claims it as one run split in lane order: every request is
+    served, in the pool and disjoint.  When it does not fit, the lanes claim
+    one by one in lane order, so each gets exactly the run the oracle's
+    gather_multiple (veb.cuh:323-452, 1160-1218) returns when replayed over
+    the same free segments, and nullptr once none is left.  Both drain back
+    to the fresh state after the frees."""
+    import torch
+    a = make(gpu, 4 << 30, det=False, limits=None)
+    rng = np.random.default_rng(8)
+    nseg = rng.integers(2, 5, 64)
+    sizes = (nseg.astype(np.uint64) - 1) * np.uint64(S) + np.uint64(1)
+    ptrs = a.malloc_bulk(torch.from_numpy(sizes.astype(np.int64)).cuda())
+    torch.cuda.synchronize()
+    p = ptrs.cpu().numpy().view(np.uint64)
+    assert np.all(p != 0)
+    H.check_ptrs(p, sizes, a.pool_base, a.num_segments * S)
+    assert np.all((p - np.uint64(a.pool_base)) % np.uint64(S) == 0)
+    a.free_bulk(ptrs[torch.randperm(64).cuda()].contiguous())
+    torch.cuda.synchronize()
+    bottom = a.tree_layers(-1)[-1]
+    ref = oracle.Veb(a.num_segments, filled=False)
+    for w, v in enumerate(bottom):
+        for b in range(64):
+            if (int(v) >> b) & 1:
+                ref.insert(w * 64 + b)
+    expect = []
+    for _ in range(64):
+        x = ref.gather_multiple(5)
+        expect.append(0 if x >= a.num_segments else a.pool_base + x * S)
+    sizes = np.full(64, 4 * S + 1, dtype=np.uint64)  # 5 segments each: 320 > free
+    ptrs = a.malloc_bulk(torch.from_numpy(sizes.astype(np.int64)).cuda())
+    torch.cuda.synchronize()
+    p = ptrs.cpu().numpy().view(np.uint64)
+    assert 0 < int((p != 0).sum()) < 64
+    assert p.tolist() == expect
+    H.check_ptrs(p, sizes, a.pool_base, a.num_segments * S)
+    a.free_bulk(ptrs)
+    torch.cuda.synchronize()
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_churn_patterns_and_counts(gpu):
     import torch
     n, rounds, seed = 1 << 20, 10, 9

@@ -1,0 +1,68 @@
+"""The block and multi-segment request stream (include/gallatin/streams.h,
+gal_big_*) on the CPU, through the oracle library, which compiles the same
+header as the GPU kernels:
+
+- the free order of every step is a bijection of [0, n): each request is
+  freed exactly once, for small, ragged and power-of-two n;
+- sizes lie in [64 KiB, 64 MiB] for the first n_small requests and in
+  [256 MiB, 2 GiB] for the rest, spread over the whole range;
+- the glibc baseline, which runs that stream on host threads, frees every
+  request it served and finds every tag intact.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KiB, MiB, GiB = 1 << 10, 1 << 20, 1 << 30
+
+
+@pytest.mark.parametrize("n_small,n_large", [(1, 0), (0, 1), (2, 0), (3, 0), (63, 1),
+                                             (64, 0), (64, 1), (1000, 0), (4096, 64),
+                                             (8192, 0), (8191, 2)])
+def test_free_order_is_a_bijection(n_small, n_large):
+    n = n_small + n_large
+    orders = []
+    for step in range(4):
+        _, order = oracle.big_stream(7, step, n_small, n_large)
+        assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint64))
+        orders.append(order)
+    if n >= 64:
+        # a shuffled order: few fixed points, and each step's order differs
+        assert np.sum(orders[0] == np.arange(n)) < n // 8
+        assert not np.array_equal(orders[0], orders[1])
+
+
+def test_sizes_cover_their_ranges():
+    sizes, _ = oracle.big_stream(11, 0, 4096, 64)
+    small, large = sizes[:4096].astype(np.float64), sizes[4096:].astype(np.float64)
+    assert small.min() >= 64 * KiB and small.max() <= 64 * MiB
+    assert large.min() >= 256 * MiB and large.max() <= 2 * GiB
+    # uniform: the mean of 4096 draws is within 2 MiB of the midpoint
+    # (standard error 0.29 MiB), and every eighth of the range is used
+    assert abs(small.mean() - (64 * KiB + 64 * MiB) / 2) < 2 * MiB
+    edges = np.linspace(64 * KiB, 64 * MiB, 9)
+    assert np.all(np.histogram(small, edges)[0] > 300)
+    s1, _ = oracle.big_stream(11, 1, 4096, 64)
+    assert not np.array_equal(sizes, s1)
+
+
+def test_glibc_baseline_frees_every_request():
+    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
+
+    class R(ctypes.Structure):
+        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
+                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
+                    ("pad", ctypes.c_uint32)]
+    lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                      ctypes.POINTER(R)]
+    r = R()
+    assert lib.gal_cpu_big_glibc(3, 0, 3, 509, 3, 4, ctypes.byref(r)) == 0
+    assert r.misses == 0 and r.bad == 0
+    assert r.pairs == 3 * 512 and r.bytes == 2 * 8 * 3 * 512 and r.threads == 4
