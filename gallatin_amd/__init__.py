@@ -414,6 +414,20 @@ class BatchStream:
         return self.offsets.ctypes.data_as(ctypes.c_void_p)
 
 
+BATCH_PLAIN, BATCH_WORKGROUP, BATCH_CACHE = 0, 1, 2
+
+
+def _batch_mode(batched):
+    """batched: False / 0 per-thread, True / 1 workgroup batches, 2 or
+    "cache" mallocs through the workgroups' LDS slice caches."""
+    if batched == "cache":
+        return BATCH_CACHE
+    mode = int(batched)
+    if mode not in (BATCH_PLAIN, BATCH_WORKGROUP, BATCH_CACHE):
+        raise ValueError("batched is 0, 1, 2 (or False, True, 'cache')")
+    return mode
+
+
 class BatchPlan:
     """gal_batch_plan: a stream's launches captured in one hipGraph."""
 
@@ -423,7 +437,7 @@ class BatchPlan:
         _check(lib().gal_batch_plan_create(
             alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(), stream.nbatches,
             stream.ptrs.data_ptr(), stream.nslots, counters.data_ptr(), threads,
-            int(bool(batched)), workgroups, ctypes.byref(self.handle)))
+            _batch_mode(batched), workgroups, ctypes.byref(self.handle)))
 
     def launch(self):
         _check(lib().gal_batch_plan_launch(self.handle, _stream()))
@@ -445,7 +459,7 @@ def run_batches(alloc, stream, counters=None, threads=0, batched=True, workgroup
     ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
     _check(lib().gal_run_batches(alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(),
                                  stream.nbatches, stream.ptrs.data_ptr(), stream.nslots,
-                                 ctr.data_ptr(), threads, int(bool(batched)), workgroups,
+                                 ctr.data_ptr(), threads, _batch_mode(batched), workgroups,
                                  _stream()))
     return ctr
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gallatin/allocators/batched.hpp"
+#include "gallatin/allocators/slice_cache.hpp"
 #include "gallatin/allocators/gallatin.hpp"
 #include "gallatin_capi.h"
 #include "internal.hpp"
@@ -161,6 +162,32 @@ __global__ __launch_bounds__(kThreads) void k_batch_batched(gal_state* s, const 
   flush_counts(ctr, c);
 }
 
+// One batch with mallocs through the workgroup's LDS slice cache
+// (slice_cache.hpp) and frees through the per-wave path.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_batch_cached(gal_state* s, const gal_batch_op* ops,
+                                                           uint64_t n, uint64_t* ptrs,
+                                                           uint64_t nslots, uint64_t* ctr,
+                                                           uint32_t spin_cap) {
+  __shared__ cache::slice_cache lds;
+  cache::cache_init(lds);
+  op_counts c;
+  const uint64_t nchunks = (n + kThreads - 1) / kThreads;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i = chunk * kThreads + threadIdx.x;
+    void* old;
+    bool want;
+    uint64_t size;
+    uint32_t slot;
+    take_op(ops, i, i < n, ptrs, nslots, c, old, want, size, slot, spin_cap, ctr);
+    if (old) detail::free_impl(s, old);
+    void* np = cache::cache_malloc(s, lds, want, size);
+    give_op(np, want, slot, ptrs, c);
+  }
+  cache::cache_flush(s, lds);
+  flush_counts(ctr, c);
+}
+
 // The same batch through the per-thread path.
 __global__ __launch_bounds__(256) void k_batch_plain(gal_state* s, const gal_batch_op* ops,
                                                      uint64_t n, uint64_t* ptrs, uint64_t nslots,
@@ -249,15 +276,19 @@ struct stream_chunk {
 
 constexpr uint32_t kChunkRounds = 4;
 
-template <int kThreads, bool kBatched>
+// kMode: 0 per-thread malloc, 1 workgroup-batched malloc (batched.hpp),
+// 2 malloc through the workgroup's LDS slice cache (slice_cache.hpp).
+template <int kThreads, int kMode>
 __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_batch_op* ops,
                                                      const stream_chunk* chunks, uint32_t nchunks,
                                                      uint32_t* queue, uint64_t* ptrs,
                                                      uint64_t nslots, uint64_t* ctr,
                                                      uint32_t spin_cap) {
   __shared__ batch::lds_state lds;
+  __shared__ cache::slice_cache sc;
   __shared__ uint32_t cur;
-  if (kBatched) batch::batch_init(lds);
+  if (kMode == 1) batch::batch_init(lds);
+  if (kMode == 2) cache::cache_init(sc);
   op_counts c;
   for (;;) {
     if (threadIdx.x == 0) cur = utils::atomic_add(queue, 1u);
@@ -276,8 +307,10 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
       // Mallocs first, published before this round waits on any free: the
       // frees of later chunks wait on exactly these entries.
       void* np = nullptr;
-      if (kBatched)
+      if (kMode == 1)
         np = batch::batch_malloc(s, want, size, lds);
+      else if (kMode == 2)
+        np = cache::cache_malloc(s, sc, want, size);
       else if (want)
         np = detail::malloc_impl(s, size);
       give_op(np, want, slot, ptrs, c);
@@ -287,6 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
       if (old) detail::free_impl(s, old);
     }
   }
+  if (kMode == 2) cache::cache_flush(s, sc);
   flush_counts(ctr, c);
 }
 
@@ -315,6 +349,15 @@ void launch_batched(gal_state* s, const gal_batch_op* ops, uint64_t n, uint64_t*
   const uint64_t chunks = (n + kThreads - 1) / kThreads;
   if (g > chunks) g = (unsigned)chunks;
   k_batch_batched<kThreads><<<g, kThreads, 0, st>>>(s, ops, n, ptrs, nslots, ctr, kStreamSpinCap);
+}
+
+template <int kThreads>
+void launch_cached(gal_state* s, const gal_batch_op* ops, uint64_t n, uint64_t* ptrs,
+                   uint64_t nslots, uint64_t* ctr, uint32_t workgroups, hipStream_t st) {
+  unsigned g = fill_grid(k_batch_cached<kThreads>, kThreads, workgroups);
+  const uint64_t chunks = (n + kThreads - 1) / kThreads;
+  if (g > chunks) g = (unsigned)chunks;
+  k_batch_cached<kThreads><<<g, kThreads, 0, st>>>(s, ops, n, ptrs, nslots, ctr, kStreamSpinCap);
 }
 
 template <int kThreads>
@@ -353,20 +396,26 @@ static int enqueue_persistent(gal_batch_plan* plan, const gal_batch_op* ops, uin
     k_stream<T, B><<<g, T, 0, st>>>(s, ops, plan->chunks, nchunks, plan->queue, ptrs, nslots, \
                                     ctr, kStreamSpinCap);                                          \
   } while (0)
-    if (!batched)
-      GAL_STREAM_LAUNCH(256, false);
+    if (batched == 0)
+      GAL_STREAM_LAUNCH(256, 0);
+    else if (batched == 1 && threads == 256)
+      GAL_STREAM_LAUNCH(256, 1);
+    else if (batched == 1 && threads == 512)
+      GAL_STREAM_LAUNCH(512, 1);
+    else if (batched == 1)
+      GAL_STREAM_LAUNCH(1024, 1);
     else if (threads == 256)
-      GAL_STREAM_LAUNCH(256, true);
+      GAL_STREAM_LAUNCH(256, 2);
     else if (threads == 512)
-      GAL_STREAM_LAUNCH(512, true);
+      GAL_STREAM_LAUNCH(512, 2);
     else
-      GAL_STREAM_LAUNCH(1024, true);
+      GAL_STREAM_LAUNCH(1024, 2);
 #undef GAL_STREAM_LAUNCH
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "stream launch");
   if (nslots) {
-    if (!batched) {
+    if (batched != 1) {
       unsigned g = fill_grid(k_table_free_plain, 256, workgroups);
       const uint64_t need = (nslots + 255) / 256;
       if (g > need) g = (unsigned)need;
@@ -394,23 +443,29 @@ static int enqueue_stream(gal_allocator* a, const gal_batch_op* ops, const uint6
     const uint64_t n = offsets[b + 1] - offsets[b];
     if (n == 0) continue;
     const gal_batch_op* o = ops + offsets[b];
-    if (!batched) {
+    if (batched == 0) {
       unsigned g = fill_grid(k_batch_plain, 256, workgroups);
       const uint64_t need = (n + 255) / 256;
       if (g > need) g = (unsigned)need;
       k_batch_plain<<<g, 256, 0, st>>>(s, o, n, ptrs, nslots, ctr, kStreamSpinCap);
-    } else if (threads == 256) {
+    } else if (batched == 1 && threads == 256) {
       launch_batched<256>(s, o, n, ptrs, nslots, ctr, workgroups, st);
-    } else if (threads == 512) {
+    } else if (batched == 1 && threads == 512) {
       launch_batched<512>(s, o, n, ptrs, nslots, ctr, workgroups, st);
-    } else {
+    } else if (batched == 1) {
       launch_batched<1024>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    } else if (threads == 256) {
+      launch_cached<256>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    } else if (threads == 512) {
+      launch_cached<512>(s, o, n, ptrs, nslots, ctr, workgroups, st);
+    } else {
+      launch_cached<1024>(s, o, n, ptrs, nslots, ctr, workgroups, st);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch launch");
   }
   if (nslots) {
-    if (!batched) {
+    if (batched != 1) {
       unsigned g = fill_grid(k_table_free_plain, 256, workgroups);
       const uint64_t need = (nslots + 255) / 256;
       if (g > need) g = (unsigned)need;
@@ -430,8 +485,10 @@ static int enqueue_stream(gal_allocator* a, const gal_batch_op* ops, const uint6
 
 static int check_plan_args(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
                            uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
-                           uint32_t* threads) {
+                           uint32_t* threads, uint32_t batched) {
   if (!a || !offsets || !ctr || (nslots && !ptrs)) return fail(GAL_ERR_INVALID, "null argument");
+  if (batched > 2)
+    return fail(GAL_ERR_INVALID, "batched is 0 (per thread), 1 (workgroup batches) or 2 (slice cache)");
   if (nbatches && offsets[nbatches] && !ops) return fail(GAL_ERR_INVALID, "null ops");
   for (uint32_t b = 0; b < nbatches; ++b)
     if (offsets[b + 1] < offsets[b]) return fail(GAL_ERR_INVALID, "offsets must not decrease");
@@ -447,7 +504,7 @@ extern "C" {
 int gal_run_batches(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
                     uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
                     uint32_t threads, uint32_t batched, uint32_t workgroups, void* stream) {
-  const int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads);
+  const int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads, batched);
   if (rc != GAL_OK) return rc;
   return enqueue_stream(a, ops, offsets, nbatches, ptrs, nslots, ctr, threads, batched,
                         workgroups, (hipStream_t)stream);
@@ -459,7 +516,7 @@ int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint6
                           gal_batch_plan** out) {
   if (!out) return fail(GAL_ERR_INVALID, "null argument");
   *out = nullptr;
-  int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads);
+  int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads, batched);
   if (rc != GAL_OK) return rc;
   // chunk table: at most kChunkRounds * threads ops, never across batches
   const uint32_t per = kChunkRounds * (batched ? threads : 256u);

@@ -395,8 +395,13 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
     got = wave::readfirst(got);
     if (got >> 32) {
       // segment fully handed out: detach it and prefetch the next one
+      // (gallatin.cuh:1611-1614).  With XCD partitions a segment whose
+      // last block came back on a foreign XCD stays formatted in its home's
+      // class tree, so the prefetch waits until that tree is empty: taking a
+      // fresh segment for every reused one would format segments without
+      // bound (a 1-block class does it on every block).
       if (wave::is_first()) veb::remove(class_tree(s, t, p), seg);
-      if (try_lock(s, lk)) {
+      if ((s->num_parts == 1 || veb::looks_empty(class_tree(s, t, p))) && try_lock(s, lk)) {
         gather_new_segment(s, t, p);
         unlock(s, lk);
       }

@@ -383,11 +383,13 @@ int gal_serving_destroy(gal_serving *s);
  * where slot was filled by an earlier batch.  Each allocation holds its slot
  * number in its first 8 bytes, checked at free (counter [2]; an op with a
  * slot >= nslots or an unknown kind also counts there and is skipped).  One
- * persistent launch per batch (the workgroup-batched path when `batched`,
- * else plain per-thread malloc/free), then one launch freeing whatever is
- * left in ptrs[0 .. nslots), so the table is empty again at the end.
- * threads: 256, 512 or 1024 (0 = 1024, batched path only); workgroups: 0 =
- * fill the device.  Counters as for the workload kernels. */
+ * persistent launch per batch, then one launch freeing whatever is left in
+ * ptrs[0 .. nslots), so the table is empty again at the end.  batched: 0
+ * plain per-thread malloc/free; 1 the workgroup-batched path (batched.hpp);
+ * 2 mallocs through each workgroup's LDS slice cache (slice_cache.hpp) and
+ * per-wave frees.  threads: 256, 512 or 1024 (0 = 1024; modes 1 and 2
+ * only); workgroups: 0 = fill the device.  Counters as for the workload
+ * kernels. */
 typedef struct gal_batch_op {
   uint64_t arg;   /* malloc: bytes */
   uint32_t kind;  /* 0 malloc, 1 free */

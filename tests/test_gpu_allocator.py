@@ -314,6 +314,34 @@ def test_wave_of_multi_segment_requests(gpu):
     a.free_on_device()
 
 
+def test_partitioned_segments_do_not_accumulate(gpu):
+    """8 GiB pool, one partition per XCD.  Rounds of 2^16 x 4 KiB requests
+    (the 1-block-per-segment class) freed in a shuffled order, so most
+    blocks complete on a foreign XCD and their segments stay formatted for
+    their home.  Such a segment is reused by its home, and taking it must
+    not also pull a fresh segment from the pool: the formatted count levels
+    off instead of growing every round, and a drain still gives back every
+    segment."""
+    import torch
+    a = make(gpu, 8 << 30, det=False, limits=None)
+    assert a.partitions > 1
+    n = 1 << 16
+    sizes = torch.full((n,), 4096, dtype=torch.int64, device="cuda")
+    fmt = []
+    for r in range(10):
+        ptrs = a.malloc_bulk(sizes)
+        torch.cuda.synchronize()
+        assert bool((ptrs != 0).all())
+        a.free_bulk(ptrs[torch.randperm(n, device="cuda")].contiguous())
+        torch.cuda.synchronize()
+        fmt.append(a.info()["segments_formatted"])
+    print("formatted segments per round", fmt)
+    assert fmt[-1] <= fmt[2] + 8, f"formatted segments keep growing: {fmt}"
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_churn_patterns_and_counts(gpu):
     import torch
     n, rounds, seed = 1 << 20, 10, 9

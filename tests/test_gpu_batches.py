@@ -1,8 +1,9 @@
-"""Batched request streams on the GPU: the workgroup-batched and the plain
-path serve the same Zipf stream, eagerly and as replays of one captured
-hipGraph; every replay issues every malloc, frees every successful one
-exactly once with its tag intact, leaves the pointer table empty, and the
-drained instance equals the fresh state."""
+"""Batched request streams on the GPU: the workgroup-batched path, the LDS
+slice-cache path (batched=2) and the plain path serve the same Zipf stream,
+eagerly and as replays of one captured hipGraph; every replay issues every
+malloc, frees every successful one exactly once with its tag intact, leaves
+the pointer table empty, and the drained instance equals the fresh state
+(for the cache: every cached slice left unused went back to its block)."""
 import numpy as np
 import pytest
 
@@ -33,7 +34,8 @@ def _check(gpu, c, stream, reps=1, misses_ok=False):
 
 @pytest.mark.parametrize("seg,mn,batched,threads", [
     (16 << 20, 16, True, 1024), (16 << 20, 16, True, 256), (16 << 20, 16, False, 256),
-    (4 << 20, 128, True, 512), (64 << 20, 16, True, 1024)])
+    (4 << 20, 128, True, 512), (64 << 20, 16, True, 1024),
+    (16 << 20, 16, 2, 1024), (16 << 20, 16, 2, 256), (4 << 20, 128, 2, 512)])
 def test_eager_stream(gpu, small_stream, seg, mn, batched, threads):
     import torch
     a = gpu.Gallatin(seg, mn, min(seg // 4096, 4096)).generate_on_device(
@@ -47,7 +49,7 @@ def test_eager_stream(gpu, small_stream, seg, mn, batched, threads):
     a.free_on_device()
 
 
-@pytest.mark.parametrize("batched", [True, False])
+@pytest.mark.parametrize("batched", [True, False, 2])
 def test_graph_replays(gpu, small_stream, batched):
     """Three replays of the captured graph, each followed by eager work on
     the same instance (an eager run of the stream) and a synchronisation;
