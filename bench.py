@@ -275,7 +275,8 @@ def zipf_sweep(args, G, torch):
         "value": best["pairs_per_s"], "unit": "pairs/s",
         "ms_per_step": best["ms_per_step"], "p50_step_ms": best["p50_step_ms"],
         "p99_step_ms": best["p99_step_ms"],
-        "best": {k: best[k] for k in ("segment_bytes", "smallest", "biggest", "threads")},
+        "best": {k: best[k] for k in ("form", "segment_bytes", "smallest", "biggest",
+                                       "threads")},
         "misses": best["misses"], "violations": best["violations"],
         "plain_same_geometry": plain[0]["pairs_per_s"] if plain else None,
         "plain_best": plain_best["pairs_per_s"],

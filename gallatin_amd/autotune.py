@@ -1,12 +1,14 @@
 """Autotuner for batched request streams.
 
-Sweeps the allocator geometry and the workgroup shape of the batched path
-over one stream and reports pairs/s for each point, beside the plain
-per-thread path at the same geometry:
+Sweeps the allocator geometry, the workgroup shape and the form of the
+batched path over one stream and reports pairs/s for each point, beside the
+plain per-thread path at the same geometry:
 
     segment bytes  {4, 16, 64} MiB
     smallest slice {16, 128} B
     workgroup      {256, 512, 1024} threads
+    batched form   workgroup batches (batched.hpp) or per-workgroup LDS
+                   slice caches (slice_cache.hpp)
 
 The biggest slice is min(segment / 4096, 4096) bytes: a segment must hold
 one block of the biggest class.  Each point replays the stream as one
@@ -23,6 +25,7 @@ from . import (CTR_BAD, CTR_COUNT, CTR_FREES_SKIPPED, CTR_MALLOCS, CTR_MISSES, C
 from . import Gallatin, GallatinError
 
 SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
+FORMS = {"workgroup": 1, "cache": 2}
 SMALLEST = (16, 128)
 THREADS = (256, 512, 1024)
 
@@ -69,9 +72,10 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
 
 
 def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
-         smallest=SMALLEST, threads=THREADS, slots_min=32, flags=0, log=None):
-    """Sweep every (segment, smallest, workgroup) point.  Returns (best,
-    table): best is the fastest batched point with no violations."""
+         smallest=SMALLEST, threads=THREADS, forms=tuple(FORMS), slots_min=32, flags=0,
+         log=None):
+    """Sweep every (segment, smallest, workgroup, form) point.  Returns
+    (best, table): best is the fastest batched point with no violations."""
     table = []
     for seg, mn in itertools.product(segments, smallest):
         big = biggest_for(seg)
@@ -80,15 +84,16 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
         alloc = Gallatin(seg, mn, big).generate_on_device(
             pool_bytes, seed, print_info=False, slots_min=slots_min, flags=flags)
         try:
-            for wg in threads:
-                r = measure(alloc, stream, True, wg, reps, warmup)
-                r.update(path="batched", segment_bytes=seg, smallest=mn, biggest=big,
-                         threads=wg)
+            for form, wg in itertools.product(forms, threads):
+                r = measure(alloc, stream, FORMS[form], wg, reps, warmup)
+                r.update(path="batched", form=form, segment_bytes=seg, smallest=mn,
+                         biggest=big, threads=wg)
                 table.append(r)
                 if log:
                     log(r)
             r = measure(alloc, stream, False, 256, reps, warmup)
-            r.update(path="plain", segment_bytes=seg, smallest=mn, biggest=big, threads=256)
+            r.update(path="plain", form="per-thread", segment_bytes=seg, smallest=mn,
+                     biggest=big, threads=256)
             table.append(r)
             if log:
                 log(r)

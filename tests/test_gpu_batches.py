@@ -72,3 +72,15 @@ def test_graph_replays(gpu, small_stream, batched):
     a.drain()
     H.check_fresh(a)
     a.free_on_device()
+
+
+def test_autotune_sweeps_both_batched_forms(gpu, small_stream):
+    """One geometry and workgroup shape: the sweep measures workgroup
+    batches, the LDS slice cache and the per-thread path, every point
+    without violations, and names a best batched point."""
+    from gallatin_amd import autotune
+    best, table = autotune.tune(small_stream, 8 << 30, seed=3, reps=1, warmup=0,
+                                segments=(16 << 20,), smallest=(16,), threads=(256,))
+    assert {r["form"] for r in table} == {"workgroup", "cache", "per-thread"}
+    assert all(r["violations"] == 0 for r in table), table
+    assert best is not None and best["path"] == "batched"
