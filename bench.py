@@ -171,14 +171,16 @@ def tree_probe(args, G, torch):
             "us_per_search_per_wave": ms * 1e3 / searches}
 
 
-def plain_small(args, G, torch, flags=0):
+def plain_small(args, G, torch, flags=0, partitions=0):
     """Plain method, small: 1 GiB pool, Gallatin<16 MiB, 16, 4096>, one
     launch of 2^20 threads each doing malloc(16), write its uint32 id, read
     it back, free; 10 launches per step. With GAL_FLAG_NO_PAYLOAD_FENCE the
     id is written and read with device-coherent accesses and the allocator
-    skips its payload fences."""
+    skips its payload fences; with GAL_FLAG_LAZY_BOOT and one partition per
+    XCD a home free releases only to its own XCD's L2 and a malloc needs no
+    acquire."""
     a = G.Gallatin(16 << 20, 16, 4096).generate_on_device(1 << 30, args.seed, print_info=False,
-                                                          flags=flags)
+                                                          flags=flags, partitions=partitions)
     ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
     n, launches, steps = 1 << 20, 10, 5
     for _ in range(launches):  # warm-up step
@@ -193,10 +195,11 @@ def plain_small(args, G, torch, flags=0):
     torch.cuda.synchronize()
     c = ctr.cpu().tolist()
     info = a.info()
+    a_parts = a.partitions
     a.free_on_device()
     secs = e0.elapsed_time(e1) / 1e3
     return {"workload": "1 GiB pool, 2^20 threads x malloc(16)/write id/read/free, "
-                        "10 launches per step",
+                        "10 launches per step", "xcd_partitions": a_parts,
             "value": c[G.CTR_PAIRS] / secs, "unit": "pairs/s",
             "ms_per_step": secs * 1e3 / steps, "misses": c[G.CTR_MISSES],
             "violations": c[G.CTR_BAD] + info["errors"]}
@@ -488,6 +491,12 @@ def main():
     # pool too small for XCD partitions)
     out["plain_small"]["coherent_payload"] = plain_small(
         args, G, torch, flags=G.GAL_FLAG_NO_PAYLOAD_FENCE)
+    # plain payload stores, slots filled on demand so that the 1 GiB pool can
+    # take one partition per XCD (the eager boot of every class in every
+    # partition needs more segments than it has)
+    xcds = max(1, torch.cuda.get_device_properties(0).multi_processor_count // 32)
+    out["plain_small"]["partitioned_lazy_boot"] = plain_small(
+        args, G, torch, flags=G.GAL_FLAG_LAZY_BOOT, partitions=xcds)
     if not args.no_big:
         out["block_multiseg"] = block_multiseg(args, G, torch)
     if not args.no_zipf:

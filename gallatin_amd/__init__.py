@@ -27,6 +27,7 @@ GAL_FLAG_DETERMINISTIC = 1
 GAL_FLAG_NO_PAYLOAD_FENCE = 2
 GAL_FLAG_POOL_HOST = 4
 GAL_FLAG_POOL_MANAGED = 8
+GAL_FLAG_LAZY_BOOT = 16
 
 CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
 CTR_TIMEOUTS, CTR_FREES_SKIPPED = 5, 6

@@ -235,6 +235,8 @@ static hipError_t free_pool(gal_allocator* a) {
 int gal_create(const gal_config* cfg, gal_allocator** out) {
   if (!cfg || !out) return fail(GAL_ERR_INVALID, "null argument");
   *out = nullptr;
+  if ((cfg->flags & GAL_FLAG_LAZY_BOOT) && (cfg->flags & GAL_FLAG_DETERMINISTIC))
+    return fail(GAL_ERR_INVALID, "the deterministic mode boots its pinned blocks eagerly");
   if ((cfg->flags & GAL_FLAG_POOL_HOST) && (cfg->flags & GAL_FLAG_POOL_MANAGED))
     return fail(GAL_ERR_INVALID, "a pool is either pinned host or managed memory, not both");
   const uint64_t S = cfg->bytes_per_segment, m = cfg->smallest, M = cfg->biggest;
@@ -381,7 +383,9 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
     k_init_blocks<<<(unsigned)((N * B + 255) / 256), 256>>>(a->dev);
     const uint64_t top_words = h.seg_tree.layer_words[h.seg_tree.num_layers - 1];
     k_fill_tree<<<(unsigned)((top_words + 255) / 256), 256>>>(h.seg_tree);
-    k_boot<<<1, 64>>>(a->dev);
+    // with GAL_FLAG_LAZY_BOOT the slots start empty and each is filled by
+    // the first request that finds it so (the refill path)
+    if (!(cfg->flags & GAL_FLAG_LAZY_BOOT)) k_boot<<<1, 64>>>(a->dev);
     if ((e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "boot");
   }
   if (rc == GAL_OK) {
@@ -446,6 +450,7 @@ int gal_drain(gal_allocator* a, void* stream) {
 
 int gal_reboot(gal_allocator* a, void* stream) {
   if (!a) return fail(GAL_ERR_INVALID, "null allocator");
+  if (a->host.flags & GAL_FLAG_LAZY_BOOT) return GAL_OK;  // slots fill on demand
   k_boot<<<1, 64, 0, (hipStream_t)stream>>>(a->dev);
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "reboot");

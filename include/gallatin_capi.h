@@ -43,6 +43,14 @@ typedef enum gal_status {
  * memory_table.cuh:543-560).  At most one of the two. */
 #define GAL_FLAG_POOL_HOST 4u
 #define GAL_FLAG_POOL_MANAGED 8u
+/* No boot: the pinned slots start empty and the first request that finds a
+ * slot empty fills it (the refill path), so segments are formatted only for
+ * the classes and XCD partitions a workload uses.  This lets a small pool
+ * run one partition per XCD (for example 1 GiB with one size class), where
+ * the eager boot of every class in every partition would not fit.  Not
+ * with GAL_FLAG_DETERMINISTIC, whose replay follows the eager boot order
+ * (boot_shared_block_container, gallatin.cuh:195-221). */
+#define GAL_FLAG_LAZY_BOOT 16u
 
 typedef struct gal_config {
   uint64_t max_bytes;          /* pool bytes; segments = max_bytes / bytes_per_segment */
@@ -129,7 +137,8 @@ int gal_get_stats(gal_allocator *a, uint64_t *out, uint32_t n);
  * Pinned slots are empty afterwards; call gal_reboot before new work. */
 int gal_drain(gal_allocator *a, void *stream);
 
-/* Re-pin boot blocks after a drain (boot_shared_block_container order). */
+/* Re-pin boot blocks after a drain (boot_shared_block_container order); a
+ * no-op for an instance created with GAL_FLAG_LAZY_BOOT. */
 int gal_reboot(gal_allocator *a, void *stream);
 
 /* Copy metadata to host buffers for checking (any pointer may be NULL):

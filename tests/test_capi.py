@@ -55,6 +55,29 @@ def test_errors_are_reported_not_fatal():
     assert b"4096" in lib.gal_last_error()
 
 
+def test_flag_constants_match_header():
+    """Every GAL_FLAG_* of gallatin_capi.h has the same value in the Python
+    package."""
+    import gallatin_amd
+    text = open(os.path.join(ROOT, "include", "gallatin_capi.h")).read()
+    flags = dict(re.findall(r"#define (GAL_FLAG_\w+) (\d+)u", text))
+    assert "GAL_FLAG_LAZY_BOOT" in flags and len(flags) >= 5
+    for name, value in flags.items():
+        assert getattr(gallatin_amd, name) == int(value), name
+
+
+def test_lazy_boot_with_deterministic_mode_is_refused():
+    """Rejected on the host, before any device call."""
+    import gallatin_amd
+    lib = gallatin_amd.lib()
+    cfg = gallatin_amd.Config()
+    lib.gal_config_default(ctypes.byref(cfg), 1 << 30, 1)
+    cfg.flags = gallatin_amd.GAL_FLAG_LAZY_BOOT | gallatin_amd.GAL_FLAG_DETERMINISTIC
+    h = ctypes.c_void_p()
+    assert lib.gal_create(ctypes.byref(cfg), ctypes.byref(h)) == -1 and not h.value
+    assert b"eagerly" in lib.gal_last_error()
+
+
 def test_product_does_not_import_oracle():
     """The shipped package never reaches into oracle/."""
     pkg = os.path.join(ROOT, "gallatin_amd")

@@ -165,15 +165,25 @@ def test_partitions_are_one_or_per_xcd(gpu):
         gpu.Gallatin(S, 16, 4096).generate_on_device(8 << 30, 1, print_info=False, partitions=4)
 
 
-@pytest.mark.parametrize("coherent", [False, True])
-def test_plain_small_config(gpu, coherent):
+@pytest.mark.parametrize("mode", ["fenced", "coherent", "partitioned"])
+def test_plain_small_config(gpu, mode):
     """1 GiB pool, 2^20 threads x 10 launches of malloc(16) / write id /
     read back / free; every pair succeeds and the pool returns to fresh.
     coherent: the instance skips its payload fences and the id is written
-    and read with device-coherent accesses."""
+    and read with device-coherent accesses.  partitioned: slots filled on
+    demand (GAL_FLAG_LAZY_BOOT) and one partition per XCD, which the eager
+    boot cannot fit in 1 GiB."""
     import torch
-    flags = gpu.GAL_FLAG_NO_PAYLOAD_FENCE if coherent else 0
-    a = gpu.Gallatin(S, 16, 4096).generate_on_device(1 << 30, 42, print_info=False, flags=flags)
+    flags, parts = 0, 0
+    if mode == "coherent":
+        flags = gpu.GAL_FLAG_NO_PAYLOAD_FENCE
+    elif mode == "partitioned":
+        flags = gpu.GAL_FLAG_LAZY_BOOT
+        parts = torch.cuda.get_device_properties(0).multi_processor_count // 32
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(1 << 30, 42, print_info=False, flags=flags,
+                                                      partitions=parts)
+    if mode == "partitioned":
+        assert a.partitions == parts > 1
     ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
     for _ in range(10):
         a.run_plain(1 << 20, 16, counters=ctr)
