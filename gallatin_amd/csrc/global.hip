@@ -6,6 +6,8 @@
 // global_allocator.hpp, which defines the device symbol global_gallatin.
 #include <hip/hip_runtime.h>
 
+#include <numeric>
+
 #include <type_traits>
 
 #include "gallatin/allocators/global_allocator.hpp"
@@ -782,12 +784,20 @@ __device__ __forceinline__ uint64_t mix3(uint64_t a, uint64_t b, uint64_t c) {
   return x ^ (x >> 32);
 }
 
-__global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t* top, uint32_t n) {
+// The free-slot stack starts as slots 0, m, 2m, ... (mod n) from the top,
+// m coprime to n and near 0.618 n.  In slot order, sessions admitted while
+// arrivals outrun ends would take consecutive fresh slots, and the few
+// waves owning those slots would do all the appends of that young cohort
+// (their vectors grow for about 13 steps) while the others idle: those
+// steps took 1.6-2.7 ms against 1.0 ms.  Spread out, every wave gets its
+// share of young sessions.
+__global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t* top, uint32_t n,
+                             uint64_t m) {
   const uint64_t i = gallatin::utils::get_tid();
   if (i < n) {
     ss[i].alive = 0;
     ss[i].id = ~0ULL;
-    free_slots[i] = (uint32_t)(n - 1 - i);
+    free_slots[i] = (uint32_t)(((uint64_t)(n - 1 - i) * m) % n);
   }
   if (i == 0) *top = (int32_t)n;
 }
@@ -995,8 +1005,10 @@ int gal_serving_create(uint32_t max_sessions, uint64_t seed, const uint32_t* tok
     gal_serving_destroy(sv);
     return hip_fail(e, "serving buffers");
   }
+  uint64_t m = ((uint64_t)max_sessions * 2654435769ULL >> 32) | 1;  // ~0.618 n
+  while (std::gcd(m, (uint64_t)max_sessions) != 1) m += 2;
   k_serve_init<<<(max_sessions + 255) / 256, 256>>>(sv->sessions, sv->free_slots, sv->free_top,
-                                                    max_sessions);
+                                                    max_sessions, m % max_sessions ? m : 1);
   if ((e = hipDeviceSynchronize()) != hipSuccess) {
     gal_serving_destroy(sv);
     return hip_fail(e, "serving init");
