@@ -11,6 +11,12 @@
         bytes (B read and B written) in the same run, so the result is in
         bytes actually moved, whatever unit the counter definition assumes.
 
+    python -m gallatin_amd.profiling distribution <kernel_trace.csv> [--last N]
+        [--match SUBSTRING]
+        Markdown table of each kernel's launch-duration percentiles (p50,
+        p90, p99, max) over its last N launches (all by default), for the
+        kernels whose name contains SUBSTRING.
+
     python -m gallatin_amd.profiling counters <counter_collection.csv>...
         [--dispatches KERNEL]
         Markdown table of the mean of every counter per kernel over its
@@ -54,6 +60,28 @@ def stats_markdown(path, title=None, command=None):
         out.append(f"| `{kernel_key(r['Name'])}` | {r['Calls']} | "
                    f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['MinNs']) / 1e6:.3f} | "
                    f"{float(r['MaxNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} % |")
+    return "\n".join(out) + "\n"
+
+
+def distribution_markdown(path, last=0, match=""):
+    by = collections.OrderedDict()
+    for r in csv.DictReader(open(path)):
+        k = kernel_key(r["Kernel_Name"])
+        if match and match not in k:
+            continue
+        by.setdefault(k, []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    out = ["| kernel | launches | p50 ms | p90 ms | p99 ms | max ms |",
+           "|---|---|---|---|---|---|"]
+    for k, v in by.items():
+        if last:
+            v = v[-last:]
+        w = sorted(v)
+
+        def q(f):
+            return w[min(len(w) - 1, int(f * (len(w) - 1) + 0.5))]
+        out.append(f"| `{k}` | {len(w)} | {q(0.5):.3f} | {q(0.9):.3f} | {q(0.99):.3f} | "
+                   f"{w[-1]:.3f} |")
     return "\n".join(out) + "\n"
 
 
@@ -143,12 +171,18 @@ def main(argv=None):
     t.add_argument("--match", default="",
                    help="key=int,... : the benchmark settings this holds for")
     t.add_argument("--out")
+    d = sub.add_parser("distribution")
+    d.add_argument("csv")
+    d.add_argument("--last", type=int, default=0)
+    d.add_argument("--match", default="")
     c = sub.add_parser("counters")
     c.add_argument("csv", nargs="+")
     c.add_argument("--dispatches")
     a = p.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.title, a.command))
+    elif a.cmd == "distribution":
+        sys.stdout.write(distribution_markdown(a.csv, a.last, a.match))
     elif a.cmd == "counters":
         sys.stdout.write(counters_markdown(a.csv, a.dispatches))
     else:
