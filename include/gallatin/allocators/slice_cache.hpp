@@ -91,7 +91,10 @@ __device__ inline uint64_t claim_range(gal_state* s, uint32_t t, uint32_t p, uin
 
 // malloc through the cache (every lane of the wave calls it; `want` says
 // whether this lane has a request).  nullptr for lanes without one and for
-// failed requests.
+// failed requests.  The lanes are grouped by class and every group's leader
+// reads and adds to its class's word in the same instructions, so a wave
+// with requests of eight classes pays two LDS round trips, not sixteen;
+// only groups that cross the end of their range claim, one after another.
 __device__ inline void* cache_malloc(gal_state* s, slice_cache& L, bool want, uint64_t size) {
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
@@ -100,63 +103,65 @@ __device__ inline void* cache_malloc(gal_state* s, slice_cache& L, bool want, ui
   const bool cached = want && t < T && !detail::deterministic(s);
   const uint32_t p = detail::wave_part(s);
   uint64_t off = kFail;  // slice offset of this lane's allocation
-  uint32_t tc = t;       // class of `off`
-  for (uint64_t pend = __ballot(cached); pend;) {
-    const int leader = wave::first(pend);
-    const uint32_t c = wave::readlane(t, leader);
-    const uint64_t grp = __ballot(cached && t == c) & pend;
-    pend &= ~grp;
+  const uint64_t pend = __ballot(cached);
+  if (pend) {
+    const uint64_t grp = wave::match(pend, cached, t);  // lanes of this lane's class
+    const int leader = cached ? wave::first(grp) : (int)lane;
+    const bool is_leader = cached && leader == (int)lane;
     const uint32_t g = wave::count(grp);
     const uint32_t rank = wave::count(grp & wave::lanes_below());
-    const bool in = (grp >> lane) & 1;
     unsigned long long old = 0;
     uint32_t added = 0;
-    if ((int)lane == leader) {
-      const unsigned long long cur = __atomic_load_n(&L.w[c], __ATOMIC_RELAXED);
+    if (is_leader) {
+      const unsigned long long cur = __atomic_load_n(&L.w[t], __ATOMIC_RELAXED);
       if (((cur >> 16) & 0xFFFFu) <= (cur & 0xFFFFu)) {
-        old = atomicAdd(&L.w[c], (unsigned long long)g << 16);
+        old = atomicAdd(&L.w[t], (unsigned long long)g << 16);
         added = 1;
       }
     }
-    if (!wave::readlane(added, leader)) continue;  // exhausted: per-wave path
-    old = wave::readlane((uint64_t)old, leader);
+    // every lane takes its leader's result (an exhausted range sends the
+    // group to the per-wave path)
+    added = wave::fetch(added, leader);
+    old = wave::fetch((uint64_t)old, leader);
+    const bool live = cached && added;
     const uint32_t blk = (uint32_t)(old >> 32);
     const uint32_t next = (uint32_t)(old >> 16) & 0xFFFFu;
-    // slices left in the range when this add landed; negative once another
+    // slices left in the range when the add landed; negative once another
     // wave has crossed its end.  A signed difference on purpose: with
     // `next < end ? end - next : 0` the compiler hoisted the unsigned
     // difference with a no-wrap flag and dropped the next <= end half of
     // the crossing test below, so two waves crossed one range.
     const int32_t room = (int32_t)(old & 0xFFFFu) - (int32_t)next;
-    const uint32_t taken = room > 0 ? min(g, (uint32_t)room) : 0u;
-    if (in && rank < taken) {
-      off = ((uint64_t)blk << 12) + next + rank;
-      tc = c;
-    }
-    if (room >= 0 && room < (int32_t)g) {
-      // this wave crossed the end: claim for its other lanes plus a stock
-      const uint32_t need = g - taken;
+    const uint32_t taken = live && room > 0 ? min(g, (uint32_t)room) : 0u;
+    if (live && rank < taken) off = ((uint64_t)blk << 12) + next + rank;
+    // groups whose add crossed the end claim for their other lanes plus a
+    // stock, one group at a time (lane order of the leaders)
+    const bool crossed = live && room >= 0 && room < (int32_t)g;
+    for (uint64_t m = __ballot(is_leader && crossed); m; m &= m - 1) {
+      const int l = wave::first(m);
+      const uint32_t c = wave::readlane(t, l);
+      const uint64_t gl = wave::readlane(grp, l);
+      const uint32_t tl = wave::readlane(taken, l);
+      const uint32_t need = wave::count(gl) - tl;
       const uint64_t r = claim_range(s, c, p, need + stock_of(c));
       const uint32_t rb = (uint32_t)(r >> 32);
       const uint32_t rs = (uint32_t)(r >> 16) & 0xFFFFu;
       const uint32_t rl = (uint32_t)r & 0xFFFFu;
       const uint32_t use = min(need, rl);
-      if (in && rank >= taken && rank - taken < use) {
-        off = ((uint64_t)rb << 12) + rs + (rank - taken);
-        tc = c;
-      }
+      if (((gl >> lane) & 1) && rank >= tl && rank - tl < use)
+        off = ((uint64_t)rb << 12) + rs + (rank - tl);
       const unsigned long long nw =
           rl > use ? ((unsigned long long)rb << 32) | ((unsigned long long)(rs + use) << 16) |
                          (rs + rl)
                    : kEmpty;
-      if ((int)lane == leader) atomicExch(&L.w[c], nw);
+      if ((int)lane == l) atomicExch(&L.w[c], nw);
     }
   }
   const bool got = off != kFail;
   // With one partition a slice may come from any XCD's past (as malloc_impl)
   if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1 && __ballot(got))
     utils::fence_acquire();
-  void* res = got ? (void*)(s->memory + detail::offset_to_bytes(s, off, tc)) : nullptr;
+  void* res = got ? (void*)(s->memory + detail::offset_to_bytes(s, off, t)) : nullptr;
   if (want && !got) res = detail::malloc_impl(s, size);
   return res;
 }
