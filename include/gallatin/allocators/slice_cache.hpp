@@ -5,10 +5,11 @@
 //
 //   (block id << 32) | (next slice << 16) | end slice
 //
-// A wave serves the requests of one class with one LDS atomic add of their
-// count to the "next" field: no barrier, no global atomic.  The one wave
-// whose add crosses "end" claims a new range from the pinned block for its
-// unserved lanes plus a stock for the workgroup (one counter atomic, the
+// A wave groups its requests by class, and each group's leader adds the
+// group's count to its class's "next" field with one LDS atomic (all
+// classes in the same instructions): no barrier, no global atomic.  The one
+// group whose add crosses "end" claims a new range from the pinned block for
+// its unserved lanes plus a stock for the workgroup (one counter atomic, the
 // reference's coalesced claim of gallatin.cuh:865-1078 / block.cuh:245-264
 // with the group widened to a stock), hands out its lanes' slices and
 // publishes the rest.  Requests that meet an exhausted range before the
@@ -20,8 +21,8 @@
 // counters read as if each cached slice had been allocated and freed.
 //
 // "next" only grows while a range is live, by at most one add per wave
-// that saw next <= end before adding (at most 16 waves x 64 lanes past
-// end), so the 16-bit field never carries into the block id.
+// and class that saw next <= end before adding (at most 16 waves x 64 lanes
+// past end), so the 16-bit field never carries into the block id.
 //
 // Frees are not cached: they take the per-wave path (block-grouped counter
 // atomics), which already issues one atomic per block per wave.
