@@ -352,10 +352,18 @@ def test_partitioned_segments_do_not_accumulate(gpu):
     a.free_on_device()
 
 
-def test_churn_patterns_and_counts(gpu):
+@pytest.mark.parametrize("lazy", [False, True])
+def test_churn_patterns_and_counts(gpu, lazy):
+    """The mixed-size churn, every class and every XCD partition at once;
+    lazy: the slots start empty and fill on first use."""
     import torch
     n, rounds, seed = 1 << 20, 10, 9
-    a = make(gpu, 8 << 30, det=False, limits=None)
+    if lazy:
+        a = gpu.Gallatin(S, 16, 4096).generate_on_device(8 << 30, 42, print_info=False,
+                                                          flags=gpu.GAL_FLAG_LAZY_BOOT)
+        assert a.partitions > 1 and np.all(a.snapshot()[1] == 0xFFFFFFFF)
+    else:
+        a = make(gpu, 8 << 30, det=False, limits=None)
     ctr = a.run_churn(seed, n, rounds)
     torch.cuda.synchronize()
     c = ctr.cpu().tolist()
