@@ -343,6 +343,32 @@ def cpu_baselines(args):
                         "kind": "glibc malloc/free",
                         "sample": f"all {args.threads} stream threads, {r.pairs} pairs "
                                   f"in {r.seconds:.2f} s, pattern errors {r.bad}"}
+    lib.gal_cpu_plain_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(R)]
+    r = R()
+    lib.gal_cpu_plain_glibc(1 << 20, 10, cores, ctypes.byref(r))
+    res["cpu_glibc_plain_small"] = {
+        "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+        "kind": "glibc malloc/free",
+        "sample": f"the plain small stream, 2^20 ids x 10 passes of malloc(16)/write id/read/"
+                  f"free, {r.pairs} pairs in {r.seconds:.2f} s, id errors {r.bad}"}
+    if not args.no_zipf:
+        import numpy as np
+        from gallatin_amd.batch_streams import zipf_stream
+        ops, off, nslots = zipf_stream(args.seed)
+        ops = np.ascontiguousarray(ops)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        lib.gal_cpu_batches_glibc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(R)]
+        r = R()
+        lib.gal_cpu_batches_glibc(ops.ctypes.data, off.ctypes.data, len(off) - 1, nslots, cores,
+                                  ctypes.byref(r))
+        res["cpu_glibc_batched_zipf"] = {
+            "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+            "kind": "glibc malloc/free",
+            "sample": f"the same Zipf stream ({len(ops)} ops in {len(off) - 1} batches, a "
+                      f"barrier between batches), {r.pairs} pairs in {r.seconds:.2f} s, tag "
+                      f"errors {r.bad}"}
     if not args.no_big:
         lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,

@@ -156,4 +156,134 @@ int gal_cpu_big_glibc(uint64_t seed, uint32_t step0, uint32_t steps, uint64_t n_
   return 0;
 }
 
+// Plain method: `launches` passes over thread ids [0, n), each doing
+// malloc(16), writing its uint32 id, reading it back and freeing, on
+// `threads` host threads (contiguous id shards).
+int gal_cpu_plain_glibc(uint64_t n, uint32_t launches, uint32_t threads, gal_cpu_result* out) {
+  if (!out) return -1;
+  if (threads == 0) threads = (uint32_t)gal_cpu_online_cores();
+  std::atomic<uint64_t> pairs{0}, misses{0}, bad{0};
+  auto work = [&](uint64_t lo, uint64_t hi) {
+    uint64_t p = 0, m = 0, b = 0;
+    for (uint32_t l = 0; l < launches; ++l) {
+      for (uint64_t tid = lo; tid < hi; ++tid) {
+        volatile uint32_t* q = static_cast<volatile uint32_t*>(malloc(16));
+        if (!q) {
+          ++m;
+          continue;
+        }
+        q[0] = (uint32_t)tid;
+        b += q[0] != (uint32_t)tid;
+        free((void*)q);
+        ++p;
+      }
+    }
+    pairs += p;
+    misses += m;
+    bad += b;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  for (uint32_t i = 0; i < threads; ++i)
+    pool.emplace_back(work, n * i / threads, n * (i + 1) / threads);
+  for (auto& t : pool) t.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  out->pairs = pairs;
+  out->misses = misses;
+  out->bad = bad;
+  out->bytes = 8 * pairs;
+  out->seconds = std::chrono::duration<double>(t1 - t0).count();
+  out->threads = threads;
+  out->pad = 0;
+  return 0;
+}
+
+// A batched request stream (the layout of gal_batch_op in gallatin_capi.h):
+// batch b is ops[offsets[b] .. offsets[b + 1]); kind 0 mallocs arg bytes
+// into table entry `slot` and tags its first word with the slot number,
+// kind 1 checks and frees the entry's pointer.  The ops of a batch are
+// split over `threads` host threads in contiguous shards, with a barrier
+// between batches (a free only names an entry of an earlier batch); what
+// is left in the table is freed at the end.
+typedef struct gal_cpu_batch_op {
+  uint64_t arg;
+  uint32_t kind;
+  uint32_t slot;
+} gal_cpu_batch_op;
+
+int gal_cpu_batches_glibc(const gal_cpu_batch_op* ops, const uint64_t* offsets,
+                          uint32_t nbatches, uint64_t nslots, uint32_t threads,
+                          gal_cpu_result* out) {
+  if (!out || !offsets || (nbatches && !ops)) return -1;
+  if (threads == 0) threads = (uint32_t)gal_cpu_online_cores();
+  std::vector<uint64_t*> table(nslots, nullptr);
+  std::atomic<uint64_t> pairs{0}, misses{0}, bad{0};
+  std::atomic<uint32_t> arrived{0}, phase{0};
+  auto barrier = [&](uint32_t& local) {
+    local ^= 1;
+    if (arrived.fetch_add(1) + 1 == threads) {
+      arrived.store(0);
+      phase.store(local);
+    } else {
+      while (phase.load() != local) std::this_thread::yield();
+    }
+  };
+  auto work = [&](uint32_t me) {
+    uint64_t p = 0, m = 0, b = 0;
+    uint32_t local = 0;
+    for (uint32_t bt = 0; bt < nbatches; ++bt) {
+      const uint64_t n = offsets[bt + 1] - offsets[bt];
+      const uint64_t lo = offsets[bt] + n * me / threads;
+      const uint64_t hi = offsets[bt] + n * (me + 1) / threads;
+      for (uint64_t i = lo; i < hi; ++i) {
+        const gal_cpu_batch_op& op = ops[i];
+        if (op.slot >= nslots) {
+          ++b;
+          continue;
+        }
+        if (op.kind == 0) {
+          uint64_t* q = static_cast<uint64_t*>(malloc(op.arg < 8 ? 8 : op.arg));
+          if (!q) {
+            ++m;
+            continue;
+          }
+          q[0] = op.slot;
+          table[op.slot] = q;
+        } else {
+          uint64_t* q = table[op.slot];
+          table[op.slot] = nullptr;
+          if (!q) continue;
+          b += q[0] != op.slot;
+          free(q);
+          ++p;
+        }
+      }
+      barrier(local);
+    }
+    for (uint64_t i = nslots * me / threads; i < nslots * (me + 1) / threads; ++i) {
+      if (!table[i]) continue;
+      b += table[i][0] != i;
+      free(table[i]);
+      table[i] = nullptr;
+      ++p;
+    }
+    pairs += p;
+    misses += m;
+    bad += b;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> pool;
+  for (uint32_t i = 0; i < threads; ++i) pool.emplace_back(work, i);
+  for (auto& t : pool) t.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  out->pairs = pairs;
+  out->misses = misses;
+  out->bad = bad;
+  out->bytes = 16 * pairs;
+  out->seconds = std::chrono::duration<double>(t1 - t0).count();
+  out->threads = threads;
+  out->pad = 0;
+  return 0;
+}
+
 }  // extern "C"

@@ -7,7 +7,9 @@ header as the GPU kernels:
 - sizes lie in [64 KiB, 64 MiB] for the first n_small requests and in
   [256 MiB, 2 GiB] for the rest, spread over the whole range;
 - the glibc baseline, which runs that stream on host threads, frees every
-  request it served and finds every tag intact.
+  request it served and finds every tag intact;
+- the glibc baselines of the plain small stream and of a batched Zipf
+  stream do the same.
 """
 import ctypes
 import os
@@ -66,3 +68,41 @@ def test_glibc_baseline_frees_every_request():
     assert lib.gal_cpu_big_glibc(3, 0, 3, 509, 3, 4, ctypes.byref(r)) == 0
     assert r.misses == 0 and r.bad == 0
     assert r.pairs == 3 * 512 and r.bytes == 2 * 8 * 3 * 512 and r.threads == 4
+
+
+def _cpu_lib():
+    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
+
+    class R(ctypes.Structure):
+        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
+                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
+                    ("pad", ctypes.c_uint32)]
+    return lib, R
+
+
+def test_glibc_plain_baseline():
+    lib, R = _cpu_lib()
+    lib.gal_cpu_plain_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(R)]
+    r = R()
+    assert lib.gal_cpu_plain_glibc(10000, 3, 4, ctypes.byref(r)) == 0
+    assert r.pairs == 30000 and r.misses == 0 and r.bad == 0 and r.threads == 4
+
+
+def test_glibc_batches_baseline():
+    """Every malloc of a small Zipf stream is freed exactly once (in a later
+    batch or at the end) with its tag intact."""
+    from gallatin_amd import batch_streams as B
+    ops, off, nslots = B.zipf_stream(11, nbatches=48, max_count=1 << 13)
+    mallocs, _ = B.check_stream(ops, off, nslots)
+    ops = np.ascontiguousarray(ops)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    lib, R = _cpu_lib()
+    lib.gal_cpu_batches_glibc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(R)]
+    r = R()
+    assert lib.gal_cpu_batches_glibc(ops.ctypes.data, off.ctypes.data, len(off) - 1, nslots, 4,
+                                     ctypes.byref(r)) == 0
+    assert r.misses == 0 and r.bad == 0
+    assert r.pairs == mallocs
