@@ -369,6 +369,35 @@ def cpu_baselines(args):
             "sample": f"the same Zipf stream ({len(ops)} ops in {len(off) - 1} batches, a "
                       f"barrier between batches), {r.pairs} pairs in {r.seconds:.2f} s, tag "
                       f"errors {r.bad}"}
+    if not args.no_serving:
+        import numpy as np
+        from gallatin_amd import serving as S
+
+        class SR(ctypes.Structure):
+            _fields_ = [("appended_words", ctypes.c_uint64), ("ended", ctypes.c_uint64),
+                        ("table_ops", ctypes.c_uint64), ("table_keys", ctypes.c_uint64),
+                        ("bad", ctypes.c_uint64), ("live", ctypes.c_uint32),
+                        ("threads", ctypes.c_uint32)]
+        lib.gal_cpu_serving_glibc.argtypes = [
+            ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+            ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+            ctypes.POINTER(SR)]
+        tok = np.ascontiguousarray(S.tokens_table(), dtype=np.uint32)
+        warm, steps = 20, 10
+        ms = np.zeros(steps)
+        r = SR()
+        lib.gal_cpu_serving_glibc(args.seed, warm, steps, 1 << 19, 2000.0, 0.01, 1 << 20,
+                                  tok.ctypes.data, cores, ms.ctypes.data, ctypes.byref(r))
+        srt = np.sort(ms)
+        res["cpu_glibc_serving"] = {
+            "value": float(srt[len(srt) // 2]), "unit": "ms per step (p50)",
+            "p99_step_ms": float(srt[-1]), "cores": cores,
+            "kind": "glibc malloc/free + std::unordered_map sharded per thread",
+            "sample": f"{warm} untimed + {steps} timed steps of the serving step (Poisson(2000) "
+                      f"arrivals, geometric(0.01) lifetimes, lognormal(3,1) x 32 words, 2^20 "
+                      f"table ops); at the end {r.live} live sessions and {r.table_keys} keys "
+                      f"(the device run is timed after 300 warm-up steps, at a larger state), "
+                      f"word errors {r.bad}"}
     if not args.no_big:
         lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,

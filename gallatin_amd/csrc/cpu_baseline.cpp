@@ -10,7 +10,9 @@
 
 #include <atomic>
 #include <chrono>
+#include <random>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "gallatin/streams.h"
@@ -283,6 +285,153 @@ int gal_cpu_batches_glibc(const gal_cpu_batch_op* ops, const uint64_t* offsets,
   out->seconds = std::chrono::duration<double>(t1 - t0).count();
   out->threads = threads;
   out->pad = 0;
+  return 0;
+}
+
+// The serving loop on host threads: the step of the device loop with glibc
+// malloc/free for the session vectors and a std::unordered_map sharded by
+// key hash (one shard per thread, so no locks) for the table.
+//   1. each live session ends with probability p_end (its chunks are freed)
+//      or appends tokens x 32 words (tokens from the 1024-entry quantile
+//      table) to a vector of chunks of 16, 16, 32, ... 4096 words, at most
+//      8192 words; word p of session id holds (id << 32) | p;
+//   2. Poisson(arrivals_mean) arrivals take free slots;
+//   3. ht_ops table operations, 70 % insert / 30 % remove of keys uniform
+//      in [1, 2^28].
+// Steps 0 .. warm-1 are untimed; step_ms receives the durations of the
+// `steps` timed steps.
+typedef struct gal_cpu_serving_result {
+  uint64_t appended_words, ended, table_ops, table_keys, bad;
+  uint32_t live, threads;
+} gal_cpu_serving_result;
+
+namespace {
+inline uint64_t srv_mix(uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t x = a * 0x9E3779B97F4A7C15ULL ^ b;
+  x ^= x >> 31;
+  x *= 0xBF58476D1CE4E5B9ULL;
+  x ^= c + 0x94D049BB133111EBULL;
+  x ^= x >> 29;
+  x *= 0x94D049BB133111EBULL;
+  return x ^ (x >> 32);
+}
+struct cpu_session {
+  uint64_t* chunk[10];
+  uint64_t id;
+  uint32_t size;
+  uint32_t alive;
+};
+}  // namespace
+
+int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t max_sessions,
+                          double arrivals_mean, double p_end, uint32_t ht_ops,
+                          const uint32_t* tokens, uint32_t threads, double* step_ms,
+                          gal_cpu_serving_result* out) {
+  if (!out || !tokens || !step_ms || max_sessions == 0) return -1;
+  if (threads == 0) threads = (uint32_t)gal_cpu_online_cores();
+  std::vector<cpu_session> ss(max_sessions);
+  for (auto& s : ss) s = cpu_session{{}, 0, 0, 0};
+  std::vector<uint32_t> free_slots(max_sessions);
+  for (uint32_t i = 0; i < max_sessions; ++i) free_slots[i] = max_sessions - 1 - i;
+  std::vector<std::unordered_map<uint64_t, uint64_t>> shard(threads);
+  const uint32_t end_q32 = (uint32_t)(p_end * 4294967296.0);
+  std::mt19937_64 rng(seed);
+  std::poisson_distribution<uint32_t> arrivals(arrivals_mean);
+  uint64_t next_id = 0;
+  std::atomic<uint64_t> words{0}, ended{0}, bad{0};
+  std::vector<std::vector<uint32_t>> ended_slots(threads);
+  for (uint32_t step = 0; step < warm + steps; ++step) {
+    const auto t0 = std::chrono::steady_clock::now();
+    // 1. session pass, contiguous slot shards
+    std::vector<std::thread> pool;
+    for (uint32_t me = 0; me < threads; ++me) {
+      pool.emplace_back([&, me]() {
+        uint64_t w = 0, e = 0, b = 0;
+        ended_slots[me].clear();
+        const uint32_t lo = (uint32_t)((uint64_t)max_sessions * me / threads);
+        const uint32_t hi = (uint32_t)((uint64_t)max_sessions * (me + 1) / threads);
+        for (uint32_t i = lo; i < hi; ++i) {
+          cpu_session& s = ss[i];
+          if (!s.alive) continue;
+          const uint64_t r = srv_mix(seed, s.id, step);
+          if ((uint32_t)r < end_q32) {
+            if (s.chunk[0]) b += s.chunk[0][0] != (s.id << 32);  // its first word
+            for (auto& c : s.chunk) {
+              free(c);
+              c = nullptr;
+            }
+            s.alive = 0;
+            ended_slots[me].push_back(i);
+            ++e;
+            continue;
+          }
+          uint64_t want = (uint64_t)tokens[(r >> 32) & 1023] * 32;
+          uint64_t at = s.size;
+          const uint64_t end = at + want > 8192 ? 8192 : at + want;
+          while (at < end) {
+            // chunk d holds words [lo_d, lo_d + 16 << max(d - 1, 0)) with
+            // lo_0 = 0, lo_d = 16 << (d - 1) for d >= 1
+            const uint32_t d = at < 16 ? 0u : (uint32_t)(63 - __builtin_clzll(at >> 4)) + 1u;
+            const uint64_t lo_d = d == 0 ? 0 : (16ULL << (d - 1));
+            const uint64_t n_d = d == 0 ? 16 : (16ULL << (d - 1));
+            if (!s.chunk[d]) s.chunk[d] = static_cast<uint64_t*>(malloc(n_d * 8));
+            uint64_t* c = s.chunk[d];
+            const uint64_t stop = lo_d + n_d < end ? lo_d + n_d : end;
+            for (uint64_t p = at; p < stop; ++p) c[p - lo_d] = (s.id << 32) | p;
+            w += stop - at;
+            at = stop;
+          }
+          s.size = (uint32_t)at;
+        }
+        words += w;
+        ended += e;
+        bad += b;
+      });
+    }
+    for (auto& t : pool) t.join();
+    pool.clear();
+    for (auto& v : ended_slots)
+      for (uint32_t i : v) free_slots.push_back(i);
+    // 2. admission
+    const uint32_t a = arrivals(rng);
+    for (uint32_t k = 0; k < a && !free_slots.empty(); ++k) {
+      const uint32_t i = free_slots.back();
+      free_slots.pop_back();
+      ss[i] = cpu_session{{}, next_id++, 0, 1};
+    }
+    // 3. table traffic, each thread serving the keys of its shard
+    for (uint32_t me = 0; me < threads; ++me) {
+      pool.emplace_back([&, me]() {
+        auto& m = shard[me];
+        for (uint32_t i = 0; i < ht_ops; ++i) {
+          const uint64_t h = srv_mix(seed ^ 0x7A11E5ULL, step, i);
+          const uint64_t key = (h & ((1ULL << 28) - 1)) + 1;
+          if (srv_mix(key, 0, 0) % threads != me) continue;
+          if ((h >> 32) % 10 < 7)
+            m[key] = h;
+          else
+            m.erase(key);
+        }
+      });
+    }
+    for (auto& t : pool) t.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    if (step >= warm) step_ms[step - warm] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  uint32_t live = 0;
+  for (auto& s : ss) {
+    live += s.alive;
+    for (auto& c : s.chunk) free(c);
+  }
+  uint64_t keys = 0;
+  for (auto& m : shard) keys += m.size();
+  out->appended_words = words;
+  out->ended = ended;
+  out->table_ops = (uint64_t)ht_ops * (warm + steps);
+  out->table_keys = keys;
+  out->bad = bad;
+  out->live = live;
+  out->threads = threads;
   return 0;
 }
 

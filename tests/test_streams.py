@@ -106,3 +106,27 @@ def test_glibc_batches_baseline():
                                      ctypes.byref(r)) == 0
     assert r.misses == 0 and r.bad == 0
     assert r.pairs == mallocs
+
+
+def test_glibc_serving_baseline():
+    """A small serving run on host threads: sessions live, append and end,
+    the table holds keys, and every ending session's first word is intact."""
+    from gallatin_amd import serving as S
+
+    class SR(ctypes.Structure):
+        _fields_ = [("appended_words", ctypes.c_uint64), ("ended", ctypes.c_uint64),
+                    ("table_ops", ctypes.c_uint64), ("table_keys", ctypes.c_uint64),
+                    ("bad", ctypes.c_uint64), ("live", ctypes.c_uint32),
+                    ("threads", ctypes.c_uint32)]
+    lib, _ = _cpu_lib()
+    lib.gal_cpu_serving_glibc.argtypes = [
+        ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+        ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+        ctypes.POINTER(SR)]
+    tok = np.ascontiguousarray(S.tokens_table(), dtype=np.uint32)
+    ms = np.zeros(3)
+    r = SR()
+    assert lib.gal_cpu_serving_glibc(5, 10, 3, 4096, 200.0, 0.05, 4096, tok.ctypes.data, 4,
+                                     ms.ctypes.data, ctypes.byref(r)) == 0
+    assert r.bad == 0 and r.live > 0 and r.ended > 0 and r.table_keys > 0
+    assert r.appended_words > 0 and r.table_ops == 13 * 4096 and np.all(ms > 0)
