@@ -98,6 +98,10 @@ def test_multi_segment_known_answers(gpu):
     (2, 4 << 30, "all", 400),
     (3, 256 << 20, "all", 300),      # small pool: exhaustion and recycling
     (4, 64 << 30, "all", 300),       # 3-level trees
+    (5, 2 << 30, "slices", 600),
+    (6, 512 << 20, "all", 500),      # exhaustion on a mid-size pool
+    (7, 16 << 30, "all", 400),
+    (8, 256 << 20, "slices", 800),   # slices only, recycling under exhaustion
 ])
 def test_stream_parity(gpu, seed, pool, mix, nsteps):
     steps, live = H.make_stream(seed, nsteps, mix=mix)
