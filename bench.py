@@ -459,6 +459,44 @@ def run_path(args, G, torch, path):
     }
 
 
+def section_roofline(pairs_per_s, bytes_per_pair, atomic_peak):
+    """Atomic and HBM roofline fractions of a section: two algorithmic
+    atomics per pair (one claim, one release) against the measured
+    one-per-line atomic ceiling, and its payload bytes against HBM peak."""
+    gbs = pairs_per_s * bytes_per_pair / 1e9
+    return {"atomics_per_s": 2.0 * pairs_per_s, "atomic_peak": atomic_peak,
+            "atomic_frac": 2.0 * pairs_per_s / atomic_peak,
+            "payload_bytes_per_pair": bytes_per_pair, "payload_gbs": gbs,
+            "hbm_peak_gbs": HBM_PEAK_GBS, "hbm_frac": gbs / HBM_PEAK_GBS}
+
+
+def add_section_rooflines(out, atomic_peak):
+    """Roofline fractions beside the absolute rates of every section."""
+    ps = out.get("plain_small")
+    if ps:
+        for sec in (ps, ps.get("coherent_payload"), ps.get("partitioned_lazy_boot")):
+            if sec:
+                sec["roofline"] = section_roofline(sec["value"], 8, atomic_peak)
+    for key, per_pair in (("block_multiseg", 16), ("batched_zipf", 16)):
+        sec = out.get(key)
+        if sec and "value" in sec:
+            sec["roofline"] = section_roofline(sec["value"], per_pair, atomic_peak)
+            if key == "batched_zipf":
+                sec["roofline_plain"] = section_roofline(sec["plain_best"], per_pair,
+                                                         atomic_peak)
+    srv = out.get("serving")
+    if srv and "words_per_s" in srv:
+        # appended words (8 B each, written once) and the table's 16-byte
+        # entries, one per operation; each allocator operation (a chunk
+        # malloc or free) is one claim or one release
+        gbs = (srv["words_per_s"] * 8 + srv["table_ops_per_s"] * 16) / 1e9
+        srv["roofline"] = {"allocator_atomics_per_s": srv["allocator_ops_per_s"],
+                           "atomic_peak": atomic_peak,
+                           "atomic_frac": srv["allocator_ops_per_s"] / atomic_peak,
+                           "payload_gbs": gbs, "hbm_peak_gbs": HBM_PEAK_GBS,
+                           "hbm_frac": gbs / HBM_PEAK_GBS}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -558,6 +596,8 @@ def main():
         out["batched_zipf"] = zipf_sweep(args, G, torch)
     if not args.no_serving:
         out["serving"] = serving_run(args, G, torch)
+    if not args.no_ceilings:
+        add_section_rooflines(out, ceil["per_line"])
     if not args.no_cpu_baseline:
         out.update(cpu_baselines(args))
     print(json.dumps(out))
