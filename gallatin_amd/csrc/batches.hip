@@ -298,14 +298,18 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     if (k >= nchunks) break;
     const stream_chunk ch = chunks[k];
     const uint32_t rounds = (ch.count + kThreads - 1) / kThreads;
+    // Every malloc of the chunk is served and published before any of its
+    // frees waits: the frees of later chunks wait on exactly these entries,
+    // and a chunk's frees only name entries of earlier batches.  Serving
+    // them round by round instead (malloc, then wait and free, per round)
+    // held a chunk's later mallocs behind its own earlier waits, so waits
+    // chained from chunk to chunk through the stream.
     for (uint32_t r = 0; r < rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
       bool want, is_free;
       uint64_t size;
       uint32_t slot;
       decode_op(ops, ch.begin + j, j < ch.count, nslots, c, want, size, slot, is_free);
-      // Mallocs first, published before this round waits on any free: the
-      // frees of later chunks wait on exactly these entries.
       void* np = nullptr;
       if (kMode == 1)
         np = batch::batch_malloc(s, want, size, lds);
@@ -314,8 +318,18 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
       else if (want)
         np = detail::malloc_impl(s, size);
       give_op(np, want, slot, ptrs, c);
-      // Frees go through the wave path in both forms: a workgroup-wide
-      // free would hold every wave behind the slowest wait.
+    }
+    // Frees go through the wave path in every form: a workgroup-wide free
+    // would hold every wave behind the slowest wait.
+    for (uint32_t r = 0; r < rounds; ++r) {
+      const uint32_t j = r * kThreads + threadIdx.x;
+      bool is_free = false;
+      uint32_t slot = 0;
+      if (j < ch.count) {
+        const gal_batch_op op = ops[ch.begin + j];
+        is_free = op.kind == 1 && op.slot < nslots;  // invalid ops were counted above
+        slot = op.slot;
+      }
       void* old = is_free ? wait_free(ptrs, slot, c, spin_cap, ctr) : nullptr;
       if (old) detail::free_impl(s, old);
     }
