@@ -107,9 +107,10 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
 
 
 def main(argv=None):
-    """python -m gallatin_amd.autotune [--seed S] [--pool-gib G] [--reps R]:
-    generate the Zipf stream, sweep, print one JSON line per point and the
-    best point last."""
+    """python -m gallatin_amd.autotune [--seed S] [--pool-gib G] [--reps R]
+    [--segments-mib M ...] [--smallest B ...] [--threads T ...] [--forms F ...]:
+    generate the Zipf stream, sweep the given points (the full grid by
+    default), print one JSON line per point and the best point last."""
     import argparse
     import json
 
@@ -124,11 +125,17 @@ def main(argv=None):
     p.add_argument("--reps", type=int, default=3)
     p.add_argument("--nbatches", type=int, default=256)
     p.add_argument("--flags", type=int, default=0)
+    p.add_argument("--segments-mib", type=int, nargs="+", default=[s >> 20 for s in SEGMENTS])
+    p.add_argument("--smallest", type=int, nargs="+", default=list(SMALLEST))
+    p.add_argument("--threads", type=int, nargs="+", default=list(THREADS))
+    p.add_argument("--forms", nargs="+", choices=tuple(FORMS), default=list(FORMS))
     a = p.parse_args(argv)
     torch.cuda.set_device(0)
     ops, off, nslots = zipf_stream(a.seed, nbatches=a.nbatches)
     stream = BatchStream(ops, off, nslots)
     best, _ = tune(stream, a.pool_gib << 30, seed=a.seed, reps=a.reps, flags=a.flags,
+                   segments=tuple(m << 20 for m in a.segments_mib), smallest=tuple(a.smallest),
+                   threads=tuple(a.threads), forms=tuple(a.forms),
                    log=lambda r: print(json.dumps(r), flush=True))
     print(json.dumps({"best": best}), flush=True)
 

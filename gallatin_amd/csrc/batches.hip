@@ -264,10 +264,10 @@ __global__ __launch_bounds__(256) void k_table_free_plain(gal_state* s, uint64_t
 // run them at once: a free waits only for its own table entry.  A chunk is
 // only taken by a running workgroup and a free's malloc lies in a chunk
 // taken earlier, so the earliest unfinished chunk never waits and the grid
-// always drains; the waits are bounded all the same (counter [5]).  Each
-// round of a chunk serves and publishes its mallocs before it waits on its
-// frees, which shortens the chain of waits through the stream; the batched
-// form batches the mallocs across the workgroup and frees per wave.
+// always drains; the waits are bounded all the same (counter [5]).  A
+// chunk serves and publishes all its mallocs before any of its frees
+// waits, which shortens the chain of waits through the stream; the batched
+// forms batch the mallocs across the workgroup and free per wave.
 struct stream_chunk {
   uint64_t begin;  // first op
   uint32_t count;  // ops

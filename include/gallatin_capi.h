@@ -414,8 +414,9 @@ int gal_run_batches(gal_allocator *a, const gal_batch_op *ops_dev,
  * stream of its own: a reset kernel, the persistent kernel, the cleanup)
  * and replayed with gal_batch_plan_launch.  Workgroups take chunks of at
  * most 4 x threads ops (never across batches) in stream order from a
- * device queue; a chunk of batch b waits until every chunk of batch b - 1
- * is done (bounded: a wait that runs out counts in counter [5]).  The
+ * device queue and serve a chunk's mallocs before its frees; a free waits
+ * only for its own table entry (bounded: a wait that runs out counts in
+ * counter [5]), so each slot is filled at most once per stream.  The
  * buffers must stay valid until the plan is destroyed. */
 typedef struct gal_batch_plan gal_batch_plan;
 int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
