@@ -5,7 +5,9 @@ batched path over one stream and reports pairs/s for each point, beside the
 plain per-thread path at the same geometry:
 
     segment bytes  {4, 16, 64} MiB
-    smallest slice {16, 128} B
+    smallest slice {16, 128, 256} B (a coarser first class rounds small
+                   requests up further: internal fragmentation for fewer
+                   classes)
     workgroup      {256, 512, 1024} threads
     batched form   workgroup batches (batched.hpp) or per-workgroup LDS
                    slice caches (slice_cache.hpp)
@@ -26,7 +28,7 @@ from . import Gallatin, GallatinError
 
 SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
 FORMS = {"workgroup": 1, "cache": 2}
-SMALLEST = (16, 128)
+SMALLEST = (16, 128, 256)
 THREADS = (256, 512, 1024)
 
 
