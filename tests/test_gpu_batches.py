@@ -35,7 +35,8 @@ def _check(gpu, c, stream, reps=1, misses_ok=False):
 @pytest.mark.parametrize("seg,mn,batched,threads", [
     (16 << 20, 16, True, 1024), (16 << 20, 16, True, 256), (16 << 20, 16, False, 256),
     (4 << 20, 128, True, 512), (64 << 20, 16, True, 1024),
-    (16 << 20, 16, 2, 1024), (16 << 20, 16, 2, 256), (4 << 20, 128, 2, 512)])
+    (16 << 20, 16, 2, 1024), (16 << 20, 16, 2, 256), (4 << 20, 128, 2, 512),
+    (64 << 20, 256, True, 1024), (64 << 20, 256, 2, 1024), (64 << 20, 256, False, 256)])
 def test_eager_stream(gpu, small_stream, seg, mn, batched, threads):
     import torch
     a = gpu.Gallatin(seg, mn, min(seg // 4096, 4096)).generate_on_device(
@@ -49,13 +50,14 @@ def test_eager_stream(gpu, small_stream, seg, mn, batched, threads):
     a.free_on_device()
 
 
+@pytest.mark.parametrize("seg,mn", [(16 << 20, 16), (64 << 20, 256)])
 @pytest.mark.parametrize("batched", [True, False, 2])
-def test_graph_replays(gpu, small_stream, batched):
+def test_graph_replays(gpu, small_stream, batched, seg, mn):
     """Three replays of the captured graph, each followed by eager work on
     the same instance (an eager run of the stream) and a synchronisation;
     every replay is checked on its own."""
     import torch
-    a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(8 << 30, 6, print_info=False)
+    a = gpu.Gallatin(seg, mn, 4096).generate_on_device(8 << 30, 6, print_info=False)
     ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
     plan = gpu.BatchPlan(a, small_stream, ctr, batched=batched)
     for rep in range(3):
