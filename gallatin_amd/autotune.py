@@ -113,13 +113,27 @@ def main(argv=None):
     [--segments-mib M ...] [--smallest B ...] [--threads T ...] [--forms F ...]:
     generate the Zipf stream, sweep the given points (the full grid by
     default), print one JSON line per point and the best point last."""
-    import argparse
     import json
 
     import torch
 
     from . import BatchStream
     from .batch_streams import zipf_stream
+
+    a = parse_args(argv)
+    torch.cuda.set_device(0)
+    ops, off, nslots = zipf_stream(a.seed, nbatches=a.nbatches)
+    stream = BatchStream(ops, off, nslots)
+    best, _ = tune(stream, a.pool_gib << 30, seed=a.seed, reps=a.reps, flags=a.flags,
+                   segments=tuple(m << 20 for m in a.segments_mib), smallest=tuple(a.smallest),
+                   threads=tuple(a.threads), forms=tuple(a.forms),
+                   log=lambda r: print(json.dumps(r), flush=True))
+    print(json.dumps({"best": best}), flush=True)
+
+
+def parse_args(argv=None):
+    """The command line of main(); the defaults are the full grid."""
+    import argparse
 
     p = argparse.ArgumentParser()
     p.add_argument("--seed", type=int, default=1234)
@@ -131,15 +145,7 @@ def main(argv=None):
     p.add_argument("--smallest", type=int, nargs="+", default=list(SMALLEST))
     p.add_argument("--threads", type=int, nargs="+", default=list(THREADS))
     p.add_argument("--forms", nargs="+", choices=tuple(FORMS), default=list(FORMS))
-    a = p.parse_args(argv)
-    torch.cuda.set_device(0)
-    ops, off, nslots = zipf_stream(a.seed, nbatches=a.nbatches)
-    stream = BatchStream(ops, off, nslots)
-    best, _ = tune(stream, a.pool_gib << 30, seed=a.seed, reps=a.reps, flags=a.flags,
-                   segments=tuple(m << 20 for m in a.segments_mib), smallest=tuple(a.smallest),
-                   threads=tuple(a.threads), forms=tuple(a.forms),
-                   log=lambda r: print(json.dumps(r), flush=True))
-    print(json.dumps({"best": best}), flush=True)
+    return p.parse_args(argv)
 
 
 if __name__ == "__main__":
