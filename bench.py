@@ -67,16 +67,45 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     p.add_argument("--serving-pool-gib", type=int, default=200)
     p.add_argument("--serving-steps", type=int, default=200)
-    p.add_argument("--serving-warmup", type=int, default=300)
+    p.add_argument("--serving-warmup", type=int, default=600)
+    p.add_argument("--serving-reserve", type=float, default=0.05,
+                   help="admission reserve: fraction of the pool's segments kept free")
+    p.add_argument("--cpu-serving-steps", type=int, default=30)
+    p.add_argument("--detail-json", default="",
+                   help="also write the full result (every section, the autotune table) here")
     return p.parse_args()
 
 
 def host_cores():
+    """Host cores for the CPU baselines: every online core this process may
+    run on (its affinity mask), limited by the cgroup CPU quota when one is
+    set (the box gives a process a share of the machine, and more threads
+    than the quota only time-slice).  Returns (used, online)."""
+    online = os.sysconf("SC_NPROCESSORS_ONLN")
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        n = online
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(quota) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return max(1, n), online
+
+
+def host_available_bytes():
+    """MemAvailable of the host, or None."""
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable:"):
+                    return int(line.split()[1]) * 1024
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def measured_traffic(kernel, args):
@@ -267,7 +296,12 @@ def zipf_sweep(args, G, torch):
         return {"error": "no batched point without violations"}
     plain = [r for r in table if r["path"] == "plain"
              and r["segment_bytes"] == best["segment_bytes"] and r["smallest"] == best["smallest"]]
-    plain_best = max((r for r in table if r["path"] == "plain"), key=lambda r: r["pairs_per_s"])
+    plain_grid = [r for r in table if r["path"] == "plain" and r["in_grid"]]
+    plain_best = max(plain_grid, key=lambda r: r["pairs_per_s"])
+    outside = [r for r in table if r["path"] == "batched" and not r["in_grid"]
+               and r["violations"] == 0]
+    best_outside = max(outside, key=lambda r: r["pairs_per_s"]) if outside else None
+    keys = ("form", "segment_bytes", "smallest", "biggest", "threads")
     out = {
         "workload": "256 batches, Zipf(1.2) counts on [1, 2^20], 60% malloc lognormal(6,2) "
                     "clipped [1, 16 MiB] / 40% free of a random live pointer, u64 tag "
@@ -275,15 +309,19 @@ def zipf_sweep(args, G, torch):
         "flags": "GAL_FLAG_NO_PAYLOAD_FENCE (tag accesses are device-coherent atomics)",
         "pool_bytes": args.zipf_pool_gib << 30,
         "ops": int(off[-1]), "mallocs": int(nslots),
+        "grid": "segment {4,16,64} MiB x smallest {16,128} B x workgroup {256,512,1024}; "
+                "smallest 256 B reported beside it",
         "value": best["pairs_per_s"], "unit": "pairs/s",
         "ms_per_step": best["ms_per_step"], "p50_step_ms": best["p50_step_ms"],
         "p99_step_ms": best["p99_step_ms"],
-        "best": {k: best[k] for k in ("form", "segment_bytes", "smallest", "biggest",
-                                       "threads")},
+        "best": {k: best[k] for k in keys},
         "misses": best["misses"], "violations": best["violations"],
         "plain_same_geometry": plain[0]["pairs_per_s"] if plain else None,
         "plain_best": plain_best["pairs_per_s"],
         "batched_over_plain": best["pairs_per_s"] / plain_best["pairs_per_s"],
+        "outside_grid": None if best_outside is None else dict(
+            {k: best_outside[k] for k in keys}, value=best_outside["pairs_per_s"],
+            ms_per_step=best_outside["ms_per_step"]),
         "table": [{k: (round(v, 1) if isinstance(v, float) else v) for k, v in r.items()}
                   for r in table],
     }
@@ -293,22 +331,25 @@ def zipf_sweep(args, G, torch):
 def serving_run(args, G, torch):
     """Serving form: 200 GiB global allocator, Poisson(2000) arrivals per
     step, geometric(0.01) lifetimes, lognormal(3, 1) tokens x 256 B appended
-    per session per step to a 128 B .. 64 KiB fixed_vector, admission gated
-    on 5 % free segments, 2^20 hash-table operations per step (70 % insert /
-    30 % remove, keys uniform in [1, 2^28])."""
+    per session per step to a chunked vector (chunks of 128 B doubling to
+    64 KiB, then more 64 KiB chunks), admission gated on a reserve of free
+    segments, 2^20 hash-table operations per step (70 % insert / 30 %
+    remove, keys uniform in [1, 2^28])."""
     from gallatin_amd import serving
     G.init_global_allocator(args.serving_pool_gib << 30, args.seed, print_info=False)
     try:
         r = serving.run(torch, steps=args.serving_steps, warmup=args.serving_warmup,
-                        seed=args.seed)
+                        seed=args.seed, reserve_fraction=args.serving_reserve, log=log)
         g = G.global_allocator()
         r["allocator_errors"] = g.info()["errors"]
     finally:
         G.free_global_allocator()
-    r["workload"] = ("200 GiB pool, Poisson(2000) arrivals/step, geometric(0.01) lifetime, "
-                     "lognormal(3,1) tokens x 256 B per session-step into 128 B..64 KiB "
-                     "vector chunks, 5% free-segment admission reserve, 2^20 table ops/step "
-                     "(70% insert / 30% remove, keys in [1, 2^28])")
+    r["workload"] = (f"{args.serving_pool_gib} GiB pool, Poisson(2000) arrivals/step, "
+                     "geometric(0.01) lifetime, lognormal(3,1) tokens x 256 B per session-step "
+                     "into chunks of 128 B doubling to 64 KiB then 64 KiB each, "
+                     f"{args.serving_reserve:.0%} free-segment admission reserve, 2^20 table "
+                     "ops/step (70% insert / 30% remove, keys in [1, 2^28]), "
+                     f"{args.serving_warmup} warm-up steps")
     r["pool_bytes"] = args.serving_pool_gib << 30
     return r
 
@@ -322,94 +363,68 @@ def cpu_baselines(args):
     pairs, misses, bad = a.run_churn_sequential(args.seed, 0, n, args.rounds)
     dt = time.perf_counter() - t0
     a.close()
+    log(f"oracle replay: {pairs / dt:.4g} pairs/s")
     res["cpu_baseline"] = {
         "value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
         "sample": f"oracle sequential replay of stream threads 0..{n - 1} "
                   f"({pairs} pairs, {args.rounds} rounds, fill+verify) in {dt:.1f} s",
     }
-    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
-
-    class R(ctypes.Structure):
-        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
-                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
-                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
-                    ("pad", ctypes.c_uint32)]
-    lib.gal_cpu_churn_glibc.argtypes = [ctypes.c_uint64] * 3 + [
-        ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(R)]
-    r = R()
-    cores = host_cores()
-    lib.gal_cpu_churn_glibc(args.seed, 0, args.threads, args.rounds, cores, ctypes.byref(r))
-    res["cpu_glibc"] = {"value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
-                        "kind": "glibc malloc/free",
-                        "sample": f"all {args.threads} stream threads, {r.pairs} pairs "
-                                  f"in {r.seconds:.2f} s, pattern errors {r.bad}"}
-    lib.gal_cpu_plain_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                        ctypes.POINTER(R)]
-    r = R()
-    lib.gal_cpu_plain_glibc(1 << 20, 10, cores, ctypes.byref(r))
+    from gallatin_amd import cpu_baseline as C
+    cores, online = host_cores()
+    r = C.churn(args.seed, 0, args.threads, args.rounds, cores)
+    res["cpu_glibc"] = {"value": r["pairs"] / r["seconds"], "unit": "pairs/s", "cores": cores,
+                        "online_cores": online, "kind": "glibc malloc/free",
+                        "sample": f"all {args.threads} stream threads, {r['pairs']} pairs "
+                                  f"in {r['seconds']:.2f} s, pattern errors {r['bad']}"}
+    r = C.plain(1 << 20, 10, cores)
     res["cpu_glibc_plain_small"] = {
-        "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+        "value": r["pairs"] / r["seconds"], "unit": "pairs/s", "cores": cores,
         "kind": "glibc malloc/free",
         "sample": f"the plain small stream, 2^20 ids x 10 passes of malloc(16)/write id/read/"
-                  f"free, {r.pairs} pairs in {r.seconds:.2f} s, id errors {r.bad}"}
+                  f"free, {r['pairs']} pairs in {r['seconds']:.2f} s, id errors {r['bad']}"}
     if not args.no_zipf:
-        import numpy as np
         from gallatin_amd.batch_streams import zipf_stream
         ops, off, nslots = zipf_stream(args.seed)
-        ops = np.ascontiguousarray(ops)
-        off = np.ascontiguousarray(off, dtype=np.uint64)
-        lib.gal_cpu_batches_glibc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
-                                              ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(R)]
-        r = R()
-        lib.gal_cpu_batches_glibc(ops.ctypes.data, off.ctypes.data, len(off) - 1, nslots, cores,
-                                  ctypes.byref(r))
+        r = C.batches(ops, off, nslots, cores)
         res["cpu_glibc_batched_zipf"] = {
-            "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+            "value": r["pairs"] / r["seconds"], "unit": "pairs/s", "cores": cores,
             "kind": "glibc malloc/free",
             "sample": f"the same Zipf stream ({len(ops)} ops in {len(off) - 1} batches, a "
-                      f"barrier between batches), {r.pairs} pairs in {r.seconds:.2f} s, tag "
-                      f"errors {r.bad}"}
+                      f"barrier between batches), {r['pairs']} pairs in {r['seconds']:.2f} s, "
+                      f"tag errors {r['bad']}"}
     if not args.no_serving:
         import numpy as np
         from gallatin_amd import serving as S
-
-        class SR(ctypes.Structure):
-            _fields_ = [("appended_words", ctypes.c_uint64), ("ended", ctypes.c_uint64),
-                        ("table_ops", ctypes.c_uint64), ("table_keys", ctypes.c_uint64),
-                        ("bad", ctypes.c_uint64), ("live", ctypes.c_uint32),
-                        ("threads", ctypes.c_uint32)]
-        lib.gal_cpu_serving_glibc.argtypes = [
-            ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
-            ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
-            ctypes.POINTER(SR)]
-        tok = np.ascontiguousarray(S.tokens_table(), dtype=np.uint32)
-        warm, steps = 20, 10
-        ms = np.zeros(steps)
-        r = SR()
-        lib.gal_cpu_serving_glibc(args.seed, warm, steps, 1 << 19, 2000.0, 0.01, 1 << 20,
-                                  tok.ctypes.data, cores, ms.ctypes.data, ctypes.byref(r))
+        pool = args.serving_pool_gib << 30
+        # the host holds what the device pool holds; keep well inside the
+        # host's memory
+        avail = host_available_bytes()
+        if avail and pool > 0.75 * avail:
+            pool = int(0.75 * avail)
+        warm, steps = args.serving_warmup, args.cpu_serving_steps
+        r, ms = C.serving(args.seed, warm, steps, 1 << 19, 2000.0, 0.01, 1 << 20,
+                          S.tokens_table(), pool, args.serving_reserve, cores)
         srt = np.sort(ms)
         res["cpu_glibc_serving"] = {
             "value": float(srt[len(srt) // 2]), "unit": "ms per step (p50)",
-            "p99_step_ms": float(srt[-1]), "cores": cores,
+            "p99_step_ms": float(srt[min(len(srt) - 1, int(round(0.99 * (len(srt) - 1))))]),
+            "cores": cores, "live_sessions": r["live"], "gb_live": r["held_bytes"] / 1e9,
+            "refused_sessions": r["refused"], "admitted_sessions": r["admitted"],
             "kind": "glibc malloc/free + std::unordered_map sharded per thread",
             "sample": f"{warm} untimed + {steps} timed steps of the serving step (Poisson(2000) "
                       f"arrivals, geometric(0.01) lifetimes, lognormal(3,1) x 32 words, 2^20 "
-                      f"table ops); at the end {r.live} live sessions and {r.table_keys} keys "
-                      f"(the device run is timed after 300 warm-up steps, at a larger state), "
-                      f"word errors {r.bad}"}
+                      f"table ops, admission while chunk bytes < {1 - args.serving_reserve:.2f} "
+                      f"x {pool / 2**30:.0f} GiB); at the end {r['live']} live sessions holding "
+                      f"{r['held_bytes'] / 1e9:.1f} GB and {r['table_keys']} keys, word errors "
+                      f"{r['bad']}"}
     if not args.no_big:
-        lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                          ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
-                                          ctypes.POINTER(R)]
-        r = R()
-        lib.gal_cpu_big_glibc(args.seed, 3, args.big_steps, 4096, 64, cores, ctypes.byref(r))
+        r = C.big(args.seed, 3, args.big_steps, 4096, 64, cores)
         res["cpu_glibc_block_multiseg"] = {
-            "value": r.pairs / r.seconds, "unit": "pairs/s", "cores": cores,
+            "value": r["pairs"] / r["seconds"], "unit": "pairs/s", "cores": cores,
             "kind": "glibc malloc/free",
-            "sample": f"the same {args.big_steps} block and multi-segment steps, {r.pairs} "
-                      f"pairs (glibc serves every request) in {r.seconds:.2f} s, tag errors "
-                      f"{r.bad}"}
+            "sample": f"the same {args.big_steps} block and multi-segment steps, {r['pairs']} "
+                      f"pairs (glibc serves every request) in {r['seconds']:.2f} s, tag errors "
+                      f"{r['bad']}"}
     return res
 
 
@@ -497,6 +512,114 @@ def add_section_rooflines(out, atomic_peak):
                            "hbm_frac": gbs / HBM_PEAK_GBS}
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """A progress line on stderr (the result lines go to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
+def _r(v, digits=4):
+    """Round floats to `digits` significant digits (ints and others pass)."""
+    if isinstance(v, float):
+        return float(f"{v:.{digits}g}")
+    if isinstance(v, dict):
+        return {k: _r(x, digits) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [_r(x, digits) for x in v]
+    return v
+
+
+def _pick(d, *keys):
+    return {k: d[k] for k in keys if d and k in d}
+
+
+def compact(out):
+    """The headline record: the driver reads the last stdout line, so it
+    carries the metric, the headline roofline and CPU baseline, the plain
+    path beside the batched one, and one small summary per workload."""
+    keep = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+            "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "p50_step_ms",
+            "p99_step_ms", "misses", "violations", "batched_over_plain")
+    c = {k: out[k] for k in keep if k in out}
+    cfg = out.get("config", {})
+    c["config"] = _pick(cfg, "workload", "path", "allocator", "pool_bytes", "threads",
+                        "rounds", "xcd_partitions")
+    c["roofline"] = _pick(out.get("roofline", {}), "bound", "achieved", "peak", "unit",
+                          "frac", "traffic", "kernel", "algorithmic_bytes_per_launch")
+    if "atomic_roofline" in out:
+        c["atomic_roofline"] = _pick(out["atomic_roofline"], "achieved", "peak", "unit",
+                                     "frac", "per_pair")
+    if "hbm_ceilings" in out:
+        c["hbm_ceilings"] = _pick(out["hbm_ceilings"], "copy_gbs", "scatter_2048_gbs",
+                                  "traffic_frac_of_copy")
+    if "plain" in out:
+        c["plain"] = _pick(out["plain"], "value", "ms_per_step", "p99_step_ms", "misses",
+                           "violations", "kernel")
+    if "cpu_baseline" in out:
+        c["cpu_baseline"] = dict(out["cpu_baseline"])
+    if "cpu_glibc" in out:
+        c["cpu_glibc"] = _pick(out["cpu_glibc"], "value", "unit", "cores", "online_cores",
+                               "kind")
+    w = {}
+    ps = out.get("plain_small")
+    if ps:
+        w["plain_small"] = dict(
+            _pick(ps, "value", "ms_per_step", "misses", "violations"),
+            atomic_frac=ps.get("roofline", {}).get("atomic_frac"),
+            coherent_payload=(ps.get("coherent_payload") or {}).get("value"),
+            partitioned_lazy_boot=(ps.get("partitioned_lazy_boot") or {}).get("value"),
+            cpu_glibc=(out.get("cpu_glibc_plain_small") or {}).get("value"))
+    bm = out.get("block_multiseg")
+    if bm:
+        w["block_multiseg"] = dict(
+            _pick(bm, "value", "ms_per_step", "p50_step_ms", "p99_step_ms", "pairs_per_step",
+                  "misses_per_step", "violations", "fresh_after_drain"),
+            cpu_glibc=(out.get("cpu_glibc_block_multiseg") or {}).get("value"))
+    bz = out.get("batched_zipf")
+    if bz:
+        w["batched_zipf"] = dict(
+            _pick(bz, "value", "ms_per_step", "p50_step_ms", "p99_step_ms", "best", "misses",
+                  "violations", "plain_best", "batched_over_plain", "outside_grid", "error"),
+            atomic_frac=bz.get("roofline", {}).get("atomic_frac"),
+            cpu_glibc=(out.get("cpu_glibc_batched_zipf") or {}).get("value"))
+    sv = out.get("serving")
+    if sv:
+        cpu = out.get("cpu_glibc_serving") or {}
+        w["serving"] = dict(
+            _pick(sv, "p50_step_ms", "p99_step_ms", "mean_step_ms", "table_ops_per_s",
+                  "words_per_s", "allocator_ops_per_s", "live_sessions", "session_words_bad",
+                  "table_keys", "refused_sessions", "words_refused", "chunk_failures",
+                  "allocator_errors", "gb_live"),
+            cpu_glibc_p50_step_ms=cpu.get("value"), cpu_glibc_p99_step_ms=cpu.get("p99_step_ms"),
+            cpu_glibc_live_sessions=cpu.get("live_sessions"))
+    c["workloads"] = w
+    return _r(c)
+
+
+def emit(out, detail_path=""):
+    """Print every section as a JSON line of its own, then the compact
+    headline record as the last line (under 4 KB)."""
+    if detail_path:
+        with open(detail_path, "w") as f:
+            json.dump(out, f)
+    for key in ("plain_small", "block_multiseg", "serving", "hbm_ceilings", "tree_search",
+                "atomic_roofline"):
+        if key in out:
+            print(json.dumps({"section": key, **out[key]}), flush=True)
+    bz = out.get("batched_zipf")
+    if bz:
+        for row in bz.get("table", []):
+            print(json.dumps({"section": "batched_zipf.autotune", **row}), flush=True)
+        print(json.dumps({"section": "batched_zipf",
+                          **{k: v for k, v in bz.items() if k != "table"}}), flush=True)
+    for key, v in out.items():
+        if key.startswith("cpu_"):
+            print(json.dumps({"section": key, **v}), flush=True)
+    print(json.dumps(compact(out)), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -510,7 +633,10 @@ def main():
 
     torch.cuda.set_device(0)
     paths = ["batched", "plain"] if args.path == "both" else [args.path]
-    res = {p: run_path(args, G, torch, p) for p in paths}
+    res = {}
+    for p in paths:
+        res[p] = run_path(args, G, torch, p)
+        log(f"churn {p}: {res[p]['value']:.4g} pairs/s")
     head = res[paths[0]]
     achieved_gbs = head["bytes_per_step"] / (head["ms_per_step"] / 1e3) / 1e9
     out = {
@@ -578,6 +704,7 @@ def main():
             hbm["traffic_frac_of_scatter_2048"] = hbm["traffic_gbs"] / hbm["scatter_2048_gbs"]
         out["hbm_ceilings"] = hbm
         out["tree_search"] = tree_probe(args, G, torch)
+        log("ceilings measured")
     out["plain_small"] = plain_small(args, G, torch)
     # the same workload with the id written and read device-coherently, so
     # the allocator needs no payload fences (an L2 write-back per free on a
@@ -590,17 +717,22 @@ def main():
     xcds = max(1, torch.cuda.get_device_properties(0).multi_processor_count // 32)
     out["plain_small"]["partitioned_lazy_boot"] = plain_small(
         args, G, torch, flags=G.GAL_FLAG_LAZY_BOOT, partitions=xcds)
+    log("plain small done")
     if not args.no_big:
         out["block_multiseg"] = block_multiseg(args, G, torch)
+        log("block and multi-segment done")
     if not args.no_zipf:
         out["batched_zipf"] = zipf_sweep(args, G, torch)
+        log("batched Zipf sweep done")
     if not args.no_serving:
         out["serving"] = serving_run(args, G, torch)
+        log("serving done")
     if not args.no_ceilings:
         add_section_rooflines(out, ceil["per_line"])
     if not args.no_cpu_baseline:
         out.update(cpu_baselines(args))
-    print(json.dumps(out))
+        log("cpu baselines done")
+    emit(out, args.detail_json)
 
 
 if __name__ == "__main__":

@@ -49,7 +49,7 @@ CAPI_SYMBOLS = [
     "gal_bench_copy",
     "gal_bench_scatter", "gal_bench_tree",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
-    "gal_batch_plan_launch", "gal_batch_plan_destroy",
+    "gal_batch_plan_launch", "gal_batch_plan_destroy", "gal_check_batch_stream",
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
     "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
     "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
@@ -140,6 +140,7 @@ def lib():
         L.gal_batch_plan_create.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32,
                                             _u32, ctypes.POINTER(_p)]
         L.gal_batch_plan_launch.argtypes = [_p, _p]
+        L.gal_check_batch_stream.argtypes = [_p, _p, _u32, _u64, ctypes.c_int]
         L.gal_batch_plan_destroy.argtypes = [_p]
         L.gal_vector_create.argtypes = [ctypes.POINTER(_p)]
         L.gal_vector_destroy.argtypes = [_p]
@@ -154,7 +155,7 @@ def lib():
         L.gal_table_stats.argtypes = [_p, _p, _p, _p, _p]
         L.gal_serving_create.argtypes = [_u32, _u64, _p, ctypes.POINTER(_p)]
         L.gal_serving_step.argtypes = [_p, _u32, _u32, _u64, _u32, _u32, _p, _p]
-        L.gal_serving_check.argtypes = [_p, _p, _p, _p]
+        L.gal_serving_check.argtypes = [_p, _p, _p, _p, _p]
         L.gal_serving_table.argtypes = [_p, _p, _p, _p]
         L.gal_serving_destroy.argtypes = [_p]
         L.gal_queue_create.argtypes = [_p, ctypes.c_int, ctypes.POINTER(_p)]
@@ -393,6 +394,17 @@ class Gallatin:
         _check(lib().gal_run_churn_batched(self.handle, seed, nthreads, rounds,
                                            ctr.data_ptr(), workgroups, threads, _stream()))
         return ctr
+
+
+def check_batch_stream(ops, offsets, nslots, persistent=True):
+    """gal_check_batch_stream on host arrays: raises GallatinError naming
+    the first op that breaks the stream rules (a slot filled again before
+    its free, a free without an earlier malloc, and with persistent=True a
+    slot filled twice in the stream)."""
+    ops = np.ascontiguousarray(ops, dtype=BATCH_OP_DTYPE)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    _check(lib().gal_check_batch_stream(ops.ctypes.data_as(_p), offsets.ctypes.data_as(_p),
+                                        len(offsets) - 1, int(nslots), int(bool(persistent))))
 
 
 class BatchStream:

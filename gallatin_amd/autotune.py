@@ -5,9 +5,9 @@ batched path over one stream and reports pairs/s for each point, beside the
 plain per-thread path at the same geometry:
 
     segment bytes  {4, 16, 64} MiB
-    smallest slice {16, 128, 256} B (a coarser first class rounds small
-                   requests up further: internal fragmentation for fewer
-                   classes)
+    smallest slice {16, 128} B, the stated grid, and 256 B beside it (a
+                   coarser first class rounds small requests up further:
+                   internal fragmentation for fewer classes)
     workgroup      {256, 512, 1024} threads
     batched form   workgroup batches (batched.hpp) or per-workgroup LDS
                    slice caches (slice_cache.hpp)
@@ -29,6 +29,7 @@ from . import Gallatin, GallatinError
 SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
 FORMS = {"workgroup": 1, "cache": 2}
 SMALLEST = (16, 128, 256)
+GRID_SMALLEST = (16, 128)  # the stated sweep; other smallest sizes are reported beside it
 THREADS = (256, 512, 1024)
 
 
@@ -77,7 +78,9 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
          smallest=SMALLEST, threads=THREADS, forms=tuple(FORMS), slots_min=32, flags=0,
          log=None):
     """Sweep every (segment, smallest, workgroup, form) point.  Returns
-    (best, table): best is the fastest batched point with no violations."""
+    (best, table): best is the fastest batched point with no violations
+    inside the stated grid (smallest slice in GRID_SMALLEST), or outside it
+    when the sweep has no such point."""
     table = []
     for seg, mn in itertools.product(segments, smallest):
         big = biggest_for(seg)
@@ -89,13 +92,13 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
             for form, wg in itertools.product(forms, threads):
                 r = measure(alloc, stream, FORMS[form], wg, reps, warmup)
                 r.update(path="batched", form=form, segment_bytes=seg, smallest=mn,
-                         biggest=big, threads=wg)
+                         biggest=big, threads=wg, in_grid=mn in GRID_SMALLEST)
                 table.append(r)
                 if log:
                     log(r)
             r = measure(alloc, stream, False, 256, reps, warmup)
             r.update(path="plain", form="per-thread", segment_bytes=seg, smallest=mn,
-                     biggest=big, threads=256)
+                     biggest=big, threads=256, in_grid=mn in GRID_SMALLEST)
             table.append(r)
             if log:
                 log(r)
@@ -104,7 +107,8 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
         finally:
             alloc.free_on_device()
     good = [r for r in table if r["path"] == "batched" and r["violations"] == 0]
-    best = max(good, key=lambda r: r["pairs_per_s"]) if good else None
+    in_grid = [r for r in good if r["in_grid"]] or good
+    best = max(in_grid, key=lambda r: r["pairs_per_s"]) if in_grid else None
     return best, table
 
 

@@ -15,6 +15,7 @@
 // kernel per operation type (tests/src/gallatin_test.cu:59-112).
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 #include "gallatin/allocators/batched.hpp"
@@ -515,6 +516,50 @@ static int check_plan_args(gal_allocator* a, const gal_batch_op* ops, const uint
 
 extern "C" {
 
+int gal_check_batch_stream(const gal_batch_op* ops, const uint64_t* offsets, uint32_t nbatches,
+                           uint64_t nslots, int persistent) {
+  if (!offsets || (nbatches && offsets[nbatches] && !ops))
+    return fail(GAL_ERR_INVALID, "null argument");
+  if (nslots > 0xFFFFFFFFull) return fail(GAL_ERR_INVALID, "at most 2^32 slots");
+  for (uint32_t b = 0; b < nbatches; ++b)
+    if (offsets[b + 1] < offsets[b]) return fail(GAL_ERR_INVALID, "offsets must not decrease");
+  // per slot: batch of its last malloc (or ~0), and whether that malloc's
+  // pointer is still in the table (freed or not)
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  std::vector<uint32_t> last(nslots, kNone);
+  std::vector<uint8_t> live(nslots, 0), used(nslots, 0);
+  for (uint32_t b = 0; b < nbatches; ++b) {
+    for (uint64_t i = offsets[b]; i < offsets[b + 1]; ++i) {
+      const gal_batch_op& op = ops[i];
+      if (op.kind > 1) return fail(GAL_ERR_INVALID, "op " + std::to_string(i) + ": unknown kind");
+      if (op.slot >= nslots)
+        return fail(GAL_ERR_INVALID, "op " + std::to_string(i) + ": slot beyond nslots");
+      const uint32_t sl = op.slot;
+      if (op.kind == 0) {
+        if (persistent && used[sl])
+          return fail(GAL_ERR_INVALID, "op " + std::to_string(i) + ": slot " +
+                                           std::to_string(sl) +
+                                           " is filled a second time (a persistent stream "
+                                           "fills each slot at most once)");
+        if (live[sl] || last[sl] == b)
+          return fail(GAL_ERR_INVALID, "op " + std::to_string(i) + ": slot " +
+                                           std::to_string(sl) +
+                                           " is filled again before its pointer was freed");
+        last[sl] = b;
+        live[sl] = 1;
+        used[sl] = 1;
+      } else {
+        if (!live[sl] || last[sl] >= b)
+          return fail(GAL_ERR_INVALID, "op " + std::to_string(i) + ": slot " +
+                                           std::to_string(sl) +
+                                           " is freed without a malloc in an earlier batch");
+        live[sl] = 0;
+      }
+    }
+  }
+  return GAL_OK;
+}
+
 int gal_run_batches(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
                     uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
                     uint32_t threads, uint32_t batched, uint32_t workgroups, void* stream) {
@@ -532,6 +577,18 @@ int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint6
   *out = nullptr;
   int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads, batched);
   if (rc != GAL_OK) return rc;
+  // The persistent launch runs a stream's chunks at once: a slot filled
+  // twice could be overwritten before its first pointer is freed, and a
+  // free whose malloc is not in an earlier batch would wait in vain.  The
+  // ops are checked on the host once, when the plan is made.
+  if (nbatches && offsets[nbatches] > offsets[0]) {
+    std::vector<gal_batch_op> host(offsets[nbatches]);
+    const hipError_t ce = hipMemcpy(host.data(), ops, host.size() * sizeof(gal_batch_op),
+                                    hipMemcpyDeviceToHost);
+    if (ce != hipSuccess) return hip_fail(ce, "copy of the ops for checking");
+    rc = gal_check_batch_stream(host.data(), offsets, nbatches, nslots, 1);
+    if (rc != GAL_OK) return rc;
+  }
   // chunk table: at most kChunkRounds * threads ops, never across batches
   const uint32_t per = kChunkRounds * (batched ? threads : 256u);
   std::vector<stream_chunk> chunks;

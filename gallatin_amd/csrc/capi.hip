@@ -364,6 +364,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   for (uint32_t t = 0; t < T * P; ++t)
     h.sub_trees[t].words = tree_base + reinterpret_cast<uint64_t>(h.sub_trees[t].words);
   a->dev = reinterpret_cast<gal_state*>(base + o_state);
+  h.table = reinterpret_cast<memory_table*>(a->dev);
 
   // the pool starts zeroed (memory_table.cuh:518; a host pool is zeroed by
   // the host, memory_table.cuh:537)

@@ -9,6 +9,7 @@
 #include <unistd.h>
 
 #include <atomic>
+#include <cstdio>
 #include <chrono>
 #include <random>
 #include <thread>
@@ -293,15 +294,19 @@ int gal_cpu_batches_glibc(const gal_cpu_batch_op* ops, const uint64_t* offsets,
 // key hash (one shard per thread, so no locks) for the table.
 //   1. each live session ends with probability p_end (its chunks are freed)
 //      or appends tokens x 32 words (tokens from the 1024-entry quantile
-//      table) to a vector of chunks of 16, 16, 32, ... 4096 words, at most
-//      8192 words; word p of session id holds (id << 32) | p;
-//   2. Poisson(arrivals_mean) arrivals take free slots;
+//      table) to a vector of chunks of 16, 16, 32, ... 8192 words, then
+//      further chunks of 8192 words (the device's chunked_vector<uint64_t,
+//      16, 8192, 512>); word p of session id holds (id << 32) | p;
+//   2. Poisson(arrivals_mean) arrivals take free slots while the chunk
+//      bytes held stay below (1 - reserve_fraction) x pool_bytes (the
+//      device gate counts free segments of the same pool instead);
 //   3. ht_ops table operations, 70 % insert / 30 % remove of keys uniform
 //      in [1, 2^28].
 // Steps 0 .. warm-1 are untimed; step_ms receives the durations of the
 // `steps` timed steps.
 typedef struct gal_cpu_serving_result {
   uint64_t appended_words, ended, table_ops, table_keys, bad;
+  uint64_t admitted, refused, held_bytes;
   uint32_t live, threads;
 } gal_cpu_serving_result;
 
@@ -315,18 +320,32 @@ inline uint64_t srv_mix(uint64_t a, uint64_t b, uint64_t c) {
   x *= 0x94D049BB133111EBULL;
   return x ^ (x >> 32);
 }
+constexpr uint32_t kSrvGrow = 11;          // chunks 0..10: 16, 16, 32, ... 8192 words
+constexpr uint64_t kSrvGrowWords = 16384;  // words in the doubling chunks
+constexpr uint64_t kSrvTail = 8192;        // words per later chunk
+constexpr uint64_t kSrvMaxWords = kSrvGrowWords + 512 * kSrvTail;
+inline uint32_t srv_chunk_of(uint64_t p) {
+  if (p >= kSrvGrowWords) return kSrvGrow + (uint32_t)((p - kSrvGrowWords) / kSrvTail);
+  return p < 16 ? 0u : (uint32_t)(63 - __builtin_clzll(p >> 4)) + 1u;
+}
+inline uint64_t srv_chunk_start(uint32_t d) {
+  return d == 0 ? 0 : (d < kSrvGrow ? 16ULL << (d - 1) : kSrvGrowWords + (d - kSrvGrow) * kSrvTail);
+}
+inline uint64_t srv_chunk_words(uint32_t d) {
+  return d <= 1 ? 16 : (d < kSrvGrow ? 16ULL << (d - 1) : kSrvTail);
+}
 struct cpu_session {
-  uint64_t* chunk[10];
+  std::vector<uint64_t*> chunk;
   uint64_t id;
-  uint32_t size;
+  uint64_t size;
   uint32_t alive;
 };
 }  // namespace
 
 int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t max_sessions,
                           double arrivals_mean, double p_end, uint32_t ht_ops,
-                          const uint32_t* tokens, uint32_t threads, double* step_ms,
-                          gal_cpu_serving_result* out) {
+                          const uint32_t* tokens, uint64_t pool_bytes, double reserve_fraction,
+                          uint32_t threads, double* step_ms, gal_cpu_serving_result* out) {
   if (!out || !tokens || !step_ms || max_sessions == 0) return -1;
   if (threads == 0) threads = (uint32_t)gal_cpu_online_cores();
   std::vector<cpu_session> ss(max_sessions);
@@ -335,10 +354,12 @@ int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t
   for (uint32_t i = 0; i < max_sessions; ++i) free_slots[i] = max_sessions - 1 - i;
   std::vector<std::unordered_map<uint64_t, uint64_t>> shard(threads);
   const uint32_t end_q32 = (uint32_t)(p_end * 4294967296.0);
+  const double limit = (1.0 - reserve_fraction) * (double)pool_bytes;
   std::mt19937_64 rng(seed);
   std::poisson_distribution<uint32_t> arrivals(arrivals_mean);
-  uint64_t next_id = 0;
+  uint64_t next_id = 0, admitted = 0, refused = 0;
   std::atomic<uint64_t> words{0}, ended{0}, bad{0};
+  std::atomic<int64_t> held{0};
   std::vector<std::vector<uint32_t>> ended_slots(threads);
   for (uint32_t step = 0; step < warm + steps; ++step) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -347,6 +368,7 @@ int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t
     for (uint32_t me = 0; me < threads; ++me) {
       pool.emplace_back([&, me]() {
         uint64_t w = 0, e = 0, b = 0;
+        int64_t h = 0;
         ended_slots[me].clear();
         const uint32_t lo = (uint32_t)((uint64_t)max_sessions * me / threads);
         const uint32_t hi = (uint32_t)((uint64_t)max_sessions * (me + 1) / threads);
@@ -355,60 +377,71 @@ int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t
           if (!s.alive) continue;
           const uint64_t r = srv_mix(seed, s.id, step);
           if ((uint32_t)r < end_q32) {
-            if (s.chunk[0]) b += s.chunk[0][0] != (s.id << 32);  // its first word
-            for (auto& c : s.chunk) {
-              free(c);
-              c = nullptr;
+            if (!s.chunk.empty()) b += s.chunk[0][0] != (s.id << 32);  // its first word
+            for (uint32_t d = 0; d < s.chunk.size(); ++d) {
+              free(s.chunk[d]);
+              h -= (int64_t)(srv_chunk_words(d) * 8);
             }
+            s.chunk.clear();
             s.alive = 0;
             ended_slots[me].push_back(i);
             ++e;
             continue;
           }
-          uint64_t want = (uint64_t)tokens[(r >> 32) & 1023] * 32;
+          const uint64_t want = (uint64_t)tokens[(r >> 32) & 1023] * 32;
           uint64_t at = s.size;
-          const uint64_t end = at + want > 8192 ? 8192 : at + want;
+          const uint64_t end = at + want > kSrvMaxWords ? kSrvMaxWords : at + want;
           while (at < end) {
-            // chunk d holds words [lo_d, lo_d + 16 << max(d - 1, 0)) with
-            // lo_0 = 0, lo_d = 16 << (d - 1) for d >= 1
-            const uint32_t d = at < 16 ? 0u : (uint32_t)(63 - __builtin_clzll(at >> 4)) + 1u;
-            const uint64_t lo_d = d == 0 ? 0 : (16ULL << (d - 1));
-            const uint64_t n_d = d == 0 ? 16 : (16ULL << (d - 1));
-            if (!s.chunk[d]) s.chunk[d] = static_cast<uint64_t*>(malloc(n_d * 8));
+            const uint32_t d = srv_chunk_of(at);
+            const uint64_t lo_d = srv_chunk_start(d), n_d = srv_chunk_words(d);
+            if (d >= s.chunk.size()) {
+              s.chunk.push_back(static_cast<uint64_t*>(malloc(n_d * 8)));
+              h += (int64_t)(n_d * 8);
+            }
             uint64_t* c = s.chunk[d];
             const uint64_t stop = lo_d + n_d < end ? lo_d + n_d : end;
             for (uint64_t p = at; p < stop; ++p) c[p - lo_d] = (s.id << 32) | p;
             w += stop - at;
             at = stop;
           }
-          s.size = (uint32_t)at;
+          s.size = at;
         }
         words += w;
         ended += e;
         bad += b;
+        held += h;
       });
     }
     for (auto& t : pool) t.join();
     pool.clear();
     for (auto& v : ended_slots)
       for (uint32_t i : v) free_slots.push_back(i);
-    // 2. admission
+    // 2. admission, gated on the bytes held
     const uint32_t a = arrivals(rng);
-    for (uint32_t k = 0; k < a && !free_slots.empty(); ++k) {
+    const bool room = (double)held.load() < limit;
+    for (uint32_t k = 0; k < a; ++k) {
+      if (!room || free_slots.empty()) {
+        if (step >= warm) ++refused;
+        continue;
+      }
       const uint32_t i = free_slots.back();
       free_slots.pop_back();
-      ss[i] = cpu_session{{}, next_id++, 0, 1};
+      ss[i].chunk.clear();
+      ss[i].id = next_id++;
+      ss[i].size = 0;
+      ss[i].alive = 1;
+      if (step >= warm) ++admitted;
     }
     // 3. table traffic, each thread serving the keys of its shard
     for (uint32_t me = 0; me < threads; ++me) {
       pool.emplace_back([&, me]() {
         auto& m = shard[me];
         for (uint32_t i = 0; i < ht_ops; ++i) {
-          const uint64_t h = srv_mix(seed ^ 0x7A11E5ULL, step, i);
-          const uint64_t key = (h & ((1ULL << 28) - 1)) + 1;
+          const uint64_t hh = srv_mix(seed ^ 0x7A11E5ULL, step, i);
+          const uint64_t key = (hh & ((1ULL << 28) - 1)) + 1;
           if (srv_mix(key, 0, 0) % threads != me) continue;
-          if ((h >> 32) % 10 < 7)
-            m[key] = h;
+          if ((hh >> 32) % 10 < 7)
+            m[key] = hh;
           else
             m.erase(key);
         }
@@ -416,12 +449,18 @@ int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t
     }
     for (auto& t : pool) t.join();
     const auto t1 = std::chrono::steady_clock::now();
-    if (step >= warm) step_ms[step - warm] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (step >= warm) step_ms[step - warm] = ms;
+    if ((step + 1) % 50 == 0 || step + 1 == warm + steps) {
+      fprintf(stderr, "cpu serving: step %u of %u, %.1f ms, %.1f GB held, %zu free slots\n",
+              step + 1, warm + steps, ms, (double)held.load() / 1e9, free_slots.size());
+      fflush(stderr);
+    }
   }
   uint32_t live = 0;
   for (auto& s : ss) {
     live += s.alive;
-    for (auto& c : s.chunk) free(c);
+    for (auto* c : s.chunk) free(c);
   }
   uint64_t keys = 0;
   for (auto& m : shard) keys += m.size();
@@ -430,6 +469,9 @@ int gal_cpu_serving_glibc(uint64_t seed, uint32_t warm, uint32_t steps, uint32_t
   out->table_ops = (uint64_t)ht_ops * (warm + steps);
   out->table_keys = keys;
   out->bad = bad;
+  out->admitted = admitted;
+  out->refused = refused;
+  out->held_bytes = (uint64_t)held.load();
   out->live = live;
   out->threads = threads;
   return 0;

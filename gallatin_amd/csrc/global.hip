@@ -13,6 +13,7 @@
 #include "gallatin/allocators/global_allocator.hpp"
 #include "gallatin/data_structs/block_queue.hpp"
 #include "gallatin/data_structs/chaining_table.hpp"
+#include "gallatin/data_structs/chunked_vector.hpp"
 #include "gallatin/data_structs/extendible_ht.hpp"
 #include "gallatin/data_structs/fixed_vector.hpp"
 #include "gallatin_capi.h"
@@ -722,20 +723,25 @@ int gal_bqueue_stats(gal_bqueue* q, uint64_t* nodes, uint64_t* retired, uint64_t
 
 // ---- serving loop ----
 //
-// Device-resident sessions, each owning a fixed_vector of 64-bit words whose
-// chunks grow from 128 B to 32 KiB (64 KiB in all) on the global allocator.
-// A step is three launches on one stream:
+// Device-resident sessions, each owning a chunked_vector of 64-bit words
+// on the global allocator: chunks of 128 B doubling to 64 KiB, then further
+// 64 KiB chunks (at most 512 of them, 32 MiB past the first 128 KiB).  A
+// step is three launches on one stream:
 //   1. session pass: every live session ends with probability
 //      end_q32 / 2^32 (its vector is freed, its slot returned), or appends
-//      tokens x 32 words (tokens drawn from a 1024-entry quantile table)
-//      with one wave per session writing consecutive words;
+//      tokens x 32 words (tokens drawn from a 1024-entry quantile table).
+//      A wave serves 64 sessions: their missing chunks are requested by the
+//      whole wave in one allocator call per round, then the wave writes the
+//      sessions' words one session after another with 16-byte stores;
 //   2. admission: the step's arrivals take free slots if the allocator's
 //      free segments exceed `reserve_segments`, else all are refused;
 //   3. table traffic: ht_ops operations on the extendible hash table, 70 %
 //      insert / 30 % remove of keys uniform in [1, 2^28].
-// Word p of session id holds (id << 32) | p.
+// Word p of session id holds (id << 32) | p.  An append is cut short only
+// when a chunk cannot be allocated or the vector is at its 4.2 M-word
+// limit (words refused).
 
-using session_vec = fixed_vector<uint64_t, 16, 8192>;
+using session_vec = gallatin::data_structs::chunked_vector<uint64_t, 16, 8192, 512>;
 
 struct serving_session {
   session_vec vec;
@@ -802,12 +808,33 @@ __global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t*
   if (i == 0) *top = (int32_t)n;
 }
 
+// Words [at, stop) of session `sid` into chunk memory `w` that starts at
+// position cs: every lane of the wave writes 16-byte pairs (chunks are at
+// least 128-byte aligned and start at even positions).
+__device__ __forceinline__ void write_words(uint64_t* w, uint64_t cs, uint64_t at, uint64_t stop,
+                                            uint64_t sid) {
+  const uint32_t lane = gallatin::wave::lane();
+  const uint64_t tag = sid << 32;
+  if (at & 1) {  // odd head word
+    if (lane == 0) w[at - cs] = tag | at;
+    ++at;
+  }
+  const uint64_t pairs = (stop - at) >> 1;
+  ulonglong2* v = reinterpret_cast<ulonglong2*>(w + (at - cs));
+  for (uint64_t j = lane; j < pairs; j += 64) {
+    const uint64_t p = at + 2 * j;
+    v[j] = make_ulonglong2(tag | p, tag | (p + 1));
+  }
+  if ((stop - at) & 1 && lane == 0) w[stop - 1 - cs] = tag | (stop - 1);
+}
+
 // Session pass: each wave takes 64 session slots at a time.
 __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uint32_t n,
                                                         uint32_t* free_slots, int32_t* top,
                                                         uint32_t* queue, const uint32_t* tokens,
                                                         uint64_t seed, uint32_t step,
                                                         uint32_t end_q32, uint64_t* ctr) {
+  gal_state* g = global_gallatin;
   const uint32_t lane = gallatin::wave::lane();
   uint64_t ended = 0, words = 0, refused = 0, chunks = 0, chunk_fail = 0, freed = 0;
   // waves take 64 slots at a time, grid-stride (the launch has about one
@@ -829,64 +856,75 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
     at0 = (int32_t)gallatin::wave::readfirst((uint32_t)at0);
     if (ends) {  // free the vector, return the slot
       serving_session& se = ss[slot];
-      for (uint32_t d = 0; d < session_vec::n_directory; ++d)
-        if (se.vec.directory[d]) ++freed;
-      se.vec.free_vector();
+      freed += se.vec.chunks_held();
+      se.vec.free_all();
       se.alive = 0;
       free_slots[at0 + (int32_t)gallatin::wave::count(em & gallatin::wave::lanes_below())] = slot;
       ++ended;
     }
-    // each lane reads its own session's size and demand; sessions already
-    // at their 8192-word cap append nothing and stay out of the loop below
+    // each lane: its session's words [at, end) of this step
     const bool live = alive && !ends;
-    const uint64_t my_at = live ? ss[slot].vec.size : 0;
-    const uint64_t my_want = live ? (uint64_t)tokens[(r >> 32) & 1023] * 32 : 0;
-    const uint64_t my_room = my_at < 8192 ? 8192 - my_at : 0;
-    const uint64_t my_k = my_want < my_room ? my_want : my_room;
-    if (live && my_k == 0) refused += my_want;
-    // appends, one session at a time, all lanes writing
-    uint64_t todo = gallatin::wave::active() & __ballot(live && my_k > 0);
-    for (; todo; todo &= todo - 1) {
-      const int l = gallatin::wave::first(todo);
-      const uint32_t sl = gallatin::wave::readlane(slot, l);
-      const uint64_t sid = gallatin::wave::readlane(id, l);
-      session_vec& v = ss[sl].vec;
-      const uint64_t want = gallatin::wave::readlane(my_want, l);
-      uint64_t at = gallatin::wave::readlane(my_at, l);
-      const uint64_t k = gallatin::wave::readlane(my_k, l);
-      const uint64_t end = at + k;
-      while (at < end) {
-        const uint32_t d = session_vec::get_directory_index(at);
-        const uint64_t cend = d == 0 ? 16 : (16ULL << d);
-        const uint64_t stop = cend < end ? cend : end;
-        uint64_t c = 0;
-        if (lane == 0) {
-          const bool had = v.directory[d] != nullptr;
-          c = reinterpret_cast<uint64_t>(v.chunk(d));
-          if (!had && c) ++chunks;
-          if (!c) ++chunk_fail;
+    session_vec* v = live ? &ss[slot].vec : nullptr;
+    const uint64_t at = live ? v->size : 0;
+    const uint64_t want = live ? (uint64_t)tokens[(r >> 32) & 1023] * 32 : 0;
+    uint64_t end = at + want;
+    if (end > session_vec::max_items) end = session_vec::max_items;
+    // missing chunks of [at, end): the wave requests one chunk (or the tail
+    // index) per lane per round, all lanes in one allocator call
+    uint32_t d = 0, dlast = 0;
+    bool need = live && end > at;
+    if (need) {
+      d = session_vec::chunk_of(at);
+      dlast = session_vec::chunk_of(end - 1);
+      while (d <= dlast && v->chunk(d) != nullptr) ++d;
+      need = d <= dlast;
+    }
+    while (__ballot(need)) {
+      const bool idx = need && v->needs_index(d);
+      const uint64_t bytes = idx ? session_vec::index_bytes
+                                 : sizeof(uint64_t) * session_vec::chunk_items(d);
+      void* p = gallatin::allocators::detail::malloc_impl(g, bytes, need);
+      if (need) {
+        if (!p) {  // this chunk cannot be had: the append stops before it
+          ++chunk_fail;
+          const uint64_t cs = session_vec::chunk_start(d);
+          end = cs > at ? cs : at;
+          need = false;
+        } else if (idx) {
+          v->set_index(static_cast<uint64_t**>(p));
+        } else {
+          v->set_chunk(d, static_cast<uint64_t*>(p));
+          ++chunks;
+          need = ++d <= dlast;
         }
-        c = gallatin::wave::readfirst(c);
-        if (!c) break;
-        uint64_t* w = reinterpret_cast<uint64_t*>(c);
-        const uint64_t off = session_vec::get_local_position(at, d);
-        for (uint64_t j = lane; j < stop - at; j += 64) w[off + j] = (sid << 32) | (at + j);
-        if (lane == 0) words += stop - at;
-        at = stop;
-      }
-      // the session's size covers exactly the words written
-      if (lane == 0) {
-        v.size = at;
-        refused += want - (at - (end - k));
       }
     }
+    if (live) refused += at + want - end;
+    // appends, one session at a time, all lanes writing
+    for (uint64_t todo = __ballot(live && end > at); todo; todo &= todo - 1) {
+      const int l = gallatin::wave::first(todo);
+      session_vec* sv = reinterpret_cast<session_vec*>(gallatin::wave::readlane((uint64_t)v, l));
+      const uint64_t sid = gallatin::wave::readlane(id, l);
+      uint64_t a = gallatin::wave::readlane(at, l);
+      const uint64_t e = gallatin::wave::readlane(end, l);
+      while (a < e) {
+        const uint32_t c = session_vec::chunk_of(a);
+        const uint64_t cs = session_vec::chunk_start(c);
+        const uint64_t ce = cs + session_vec::chunk_items(c);
+        const uint64_t stop = ce < e ? ce : e;
+        write_words(sv->chunk(c), cs, a, stop, sid);
+        a = stop;
+      }
+      if (lane == 0) sv->size = e;
+    }
+    if (live) words += end - at;
   }
-  const uint64_t v[6] = {ended, words, refused, chunks, chunk_fail, freed};
+  const uint64_t vals[6] = {ended, words, refused, chunks, chunk_fail, freed};
   const int idx[6] = {kSrvEnded, kSrvWords, kSrvWordsRefused, kSrvChunks, kSrvChunkFailed,
                       kSrvChunksFreed};
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    uint64_t t = v[i];
+    uint64_t t = vals[i];
     for (int off = 32; off >= 1; off >>= 1) t += gallatin::wave::fetch(t, (int)(lane ^ (uint32_t)off));
     if (lane == 0 && t) gallatin::utils::atomic_add(&ctr[idx[i]], t);
   }
@@ -956,29 +994,54 @@ __global__ void k_serve_table(table_large* t, uint64_t seed, uint32_t step, uint
   count_wave(&ctr[kSrvHtAbsent], !ins && !ok);
 }
 
-__global__ void k_serve_check(serving_session* ss, uint32_t n, unsigned long long* out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !ss[i].alive) return;
-  session_vec& v = ss[i].vec;
-  const uint64_t size = v.size < 8192 ? v.size : 8192;
-  uint64_t bad = 0;
-  for (uint64_t p = 0; p < size; ++p) {
-    const uint32_t d = session_vec::get_directory_index(p);
-    if (!v.directory[d]) {
-      bad += 1;
-      continue;
+// One wave per live session (grid-stride): every word read back.  out[0]
+// bad words, [1] live sessions, [2] words, [3] chunk bytes held.
+__global__ __launch_bounds__(256) void k_serve_check(serving_session* ss, uint32_t n,
+                                                     unsigned long long* out) {
+  const uint32_t lane = gallatin::wave::lane();
+  const uint32_t wave_id = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  uint64_t bad = 0, live = 0, words = 0, held = 0;
+  for (uint32_t i = wave_id; i < n; i += nwaves) {
+    if (!ss[i].alive) continue;
+    session_vec& v = ss[i].vec;
+    const uint64_t tag = ss[i].id << 32;
+    const uint64_t size = v.size;
+    ++live;
+    words += size;
+    for (uint32_t c = 0; c < session_vec::n_chunks; ++c) {
+      const uint64_t cs = session_vec::chunk_start(c);
+      const uint64_t* w = v.chunk(c);
+      if (w) held += session_vec::chunk_items(c) * sizeof(uint64_t);
+      if (cs >= size) {
+        if (!w) break;  // chunks are installed in order
+        continue;
+      }
+      const uint64_t ce = cs + session_vec::chunk_items(c);
+      const uint64_t stop = ce < size ? ce : size;
+      if (!w) {
+        bad += stop - cs;
+        continue;
+      }
+      uint64_t b = 0;
+      for (uint64_t p = cs + lane; p < stop; p += 64) b += w[p - cs] != (tag | p);
+      for (int off = 32; off >= 1; off >>= 1) b += gallatin::wave::fetch(b, (int)(lane ^ (uint32_t)off));
+      bad += b;
     }
-    bad += v[p] != ((ss[i].id << 32) | p);
+    if (v.tail) held += session_vec::index_bytes;
   }
-  atomicAdd(&out[0], (unsigned long long)bad);
-  atomicAdd(&out[1], 1ull);
-  atomicAdd(&out[2], (unsigned long long)size);
+  if (lane == 0 && (bad | live | words | held)) {
+    atomicAdd(&out[0], (unsigned long long)bad);
+    atomicAdd(&out[1], (unsigned long long)live);
+    atomicAdd(&out[2], (unsigned long long)words);
+    atomicAdd(&out[3], (unsigned long long)held);
+  }
 }
 
 __global__ void k_serve_free_all(serving_session* ss, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !ss[i].alive) return;
-  ss[i].vec.free_vector();
+  ss[i].vec.free_all();
   ss[i].alive = 0;
 }
 
@@ -1049,21 +1112,27 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
   return e == hipSuccess ? GAL_OK : hip_fail(e, "serving step");
 }
 
-// Live sessions, words checked, mismatching words.  Synchronous.
-int gal_serving_check(gal_serving* sv, uint64_t* live, uint64_t* words, uint64_t* bad) {
+// Live sessions, words checked, mismatching words, chunk bytes held.
+// Synchronous.
+int gal_serving_check(gal_serving* sv, uint64_t* live, uint64_t* words, uint64_t* bad,
+                      uint64_t* held_bytes) {
   if (!sv) return fail(GAL_ERR_INVALID, "null argument");
   unsigned long long* c = nullptr;
-  hipError_t e = hipMalloc(&c, 3 * sizeof(unsigned long long));
+  hipError_t e = hipMalloc(&c, 4 * sizeof(unsigned long long));
   if (e != hipSuccess) return hip_fail(e, "check buffer");
-  (void)hipMemset(c, 0, 3 * sizeof(unsigned long long));
-  k_serve_check<<<(sv->max_sessions + 255) / 256, 256>>>(sv->sessions, sv->max_sessions, c);
-  unsigned long long h[3] = {0, 0, 0};
+  (void)hipMemset(c, 0, 4 * sizeof(unsigned long long));
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  k_serve_check<<<(unsigned)cus * 8, 256>>>(sv->sessions, sv->max_sessions, c);
+  unsigned long long h[4] = {0, 0, 0, 0};
   e = hipMemcpy(h, c, sizeof(h), hipMemcpyDeviceToHost);
   (void)hipFree(c);
   if (e != hipSuccess) return hip_fail(e, "check copy");
   if (bad) *bad = h[0];
   if (live) *live = h[1];
   if (words) *words = h[2];
+  if (held_bytes) *held_bytes = h[3];
   return GAL_OK;
 }
 

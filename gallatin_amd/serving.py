@@ -3,8 +3,10 @@
 Workload: arrivals per step ~ Poisson(arrivals_mean); a session lives a
 geometric(p_end) number of steps; each step a live session appends
 lognormal(mu, sigma) tokens x 256 B (32 words of 8 B each) to its
-fixed_vector (128 B .. 64 KiB chunks); sessions are admitted only while the
-allocator has more than `reserve_segments` free segments; beside them
+chunked_vector (chunks of 128 B doubling to 64 KiB, then further 64 KiB
+chunks, so an append is never cut short for want of capacity); sessions are
+admitted only while the allocator has more than `reserve_segments` free
+segments; beside them
 ht_ops operations per step (70 % insert / 30 % remove, keys uniform in
 [1, 2^28]) run on an extendible hash table that grows on the same
 allocator.  Step latency is measured with events around each step.
@@ -41,9 +43,9 @@ class Serving:
                                       _stream()))
 
     def check(self):
-        v = [ctypes.c_uint64() for _ in range(3)]
+        v = [ctypes.c_uint64() for _ in range(4)]
         _check(lib().gal_serving_check(self.handle, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("live", "words", "bad"), (x.value for x in v)))
+        return dict(zip(("live", "words", "bad", "held_bytes"), (x.value for x in v)))
 
     def table(self):
         v = [ctypes.c_uint64() for _ in range(3)]
@@ -57,11 +59,14 @@ class Serving:
 
 
 def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
-        p_end=0.01, ht_ops=1 << 20, reserve_fraction=0.05, num_segments=None):
+        p_end=0.01, ht_ops=1 << 20, reserve_fraction=0.05, num_segments=None, log=None,
+        trace_every=50):
     """Run warmup + steps steps on the current global allocator; returns
-    per-step latencies of the timed steps and the counters of all steps."""
+    per-step latencies of the timed steps, the counters of all steps, and a
+    trace of the warm-up every `trace_every` steps (free segments, live
+    sessions, admissions and refusals so far), also passed to `log`."""
+    from . import global_allocator
     if num_segments is None:
-        from . import global_allocator
         num_segments = global_allocator().num_segments
     rng = np.random.default_rng(seed)
     arrivals = rng.poisson(arrivals_mean, warmup + steps)
@@ -69,8 +74,17 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
     sv = Serving(max_sessions, seed)
     ctr = torch.zeros(16, dtype=torch.int64, device="cuda")
     try:
+        trace = []
         for k in range(warmup):
             sv.step(k, int(arrivals[k]), reserve, p_end, ht_ops, ctr)
+            if trace_every and (k + 1) % trace_every == 0:
+                c = ctr.cpu().tolist()
+                t = {"step": k + 1, "segments_free": global_allocator().info()["segments_free"],
+                     "live": c[ADMITTED] - c[ENDED], "admitted": c[ADMITTED],
+                     "refused": c[REFUSED], "chunk_failed": c[CHUNK_FAILED]}
+                trace.append(t)
+                if log:
+                    log(f"serving warm-up {t}")
         torch.cuda.synchronize()
         warm = ctr.cpu().tolist()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
@@ -84,12 +98,13 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
         total = evs[0].elapsed_time(evs[-1]) / 1e3
         chk = sv.check()
         tab = sv.table()
+        segments_free = global_allocator().info()["segments_free"]
     finally:
         sv.destroy()
     timed = [c[i] - warm[i] for i in range(16)]
     srt = sorted(ms)
     return {
-        "steps": steps, "warmup": warmup,
+        "steps": steps, "warmup": warmup, "reserve_segments": reserve, "trace": trace,
         "p50_step_ms": srt[len(srt) // 2],
         "p99_step_ms": srt[min(len(srt) - 1, int(round(0.99 * (len(srt) - 1))))],
         "mean_step_ms": total * 1e3 / steps,
@@ -98,8 +113,12 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
         "words_per_s": timed[WORDS] / total,
         "sessions_admitted_per_s": timed[ADMITTED] / total,
         "live_sessions": chk["live"], "session_words_bad": chk["bad"],
+        "session_words": chk["words"], "gb_live": chk["held_bytes"] / 1e9,
+        "segments_free_end": segments_free,
         "table_keys": tab["keys"], "table_inserts_refused": c[HT_FAILED],
-        "refused_sessions": c[REFUSED], "chunk_failures": c[CHUNK_FAILED],
+        # refusals of the timed steps (the admission gate in steady state)
+        "refused_sessions": timed[REFUSED], "admitted_sessions": timed[ADMITTED],
+        "words_refused": c[WORDS_REFUSED], "chunk_failures": c[CHUNK_FAILED],
         "counters": {n: c[i] for i, n in enumerate(
             ("admitted", "refused", "ended", "words", "words_refused", "chunks",
              "chunk_failed", "chunks_freed", "ht_inserted", "ht_removed", "ht_absent",

@@ -16,9 +16,14 @@
 //     other waves.
 //
 // Requests the batch cannot serve (whole blocks, multi-segment runs, runs of
-// 4 KiB slices, deterministic mode, or a class that needs more than kRanges
-// claims) go through the per-wave path of gallatin.hpp, so results and
-// failure behaviour are those of malloc / free.
+// 4 KiB slices, or a class that needs more than kRanges claims) go through
+// the per-wave path of gallatin.hpp, so results and failure behaviour are
+// those of malloc / free.  In the deterministic mode the class claims take
+// slot 0 and smallest-id segments like the per-wave path, so the slices a
+// batch hands out per class are exactly those of one coalesced claim of that
+// many consecutive slices (gallatin.cuh:865-1078); which request gets which
+// one depends on scheduling.  Frees of the deterministic mode take the
+// per-wave path.
 //
 // Every thread of the workgroup must call these functions (workgroup-uniform
 // control flow; blockDim.x a multiple of 64 and at most 1024), and the LDS
@@ -60,6 +65,7 @@ __device__ inline void batch_init(lds_state& L) {
 // order until served, out of ranges or out of attempts.
 __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
   const uint32_t T = s->num_trees;
+  const bool det = detail::deterministic(s);
   const uint32_t c = wave::lane();
   const uint32_t need = c < T ? L.count[c] : 0u;
   if (c < T) L.count[c] = 0;  // ready for the next batch
@@ -74,7 +80,7 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
     for (uint64_t m = __ballot(open && (uint32_t)sb == kNullBlock); m; m &= m - 1) {
       const int l = wave::first(m);
       const uint32_t sl = (uint32_t)(wave::readlane(sb, l) >> 32);
-      const uint32_t got = detail::refill_slot(s, (uint32_t)l, p, sl, false);
+      const uint32_t got = detail::refill_slot(s, (uint32_t)l, p, sl, det);
       if ((int)c == l) sb = ((uint64_t)sl << 32) | got;
     }
     const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
@@ -104,7 +110,7 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
     for (uint64_t m = rep; m; m &= m - 1) {
       const int l = wave::first(m);
       detail::replace_block(s, wave::readlane(c, l), p, wave::readlane(slot, l),
-                            wave::readlane(blk, l), false);
+                            wave::readlane(blk, l), det);
     }
     if (__ballot(waiting)) utils::backoff();
     if (open && (have || waiting) &&
@@ -127,7 +133,7 @@ struct malloc_req {
 __device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint64_t size,
                                                    lds_state& L) {
   malloc_req r{0u, 0u, false};
-  if (want && !detail::deterministic(s)) {
+  if (want) {
     r.t = size < s->smallest
               ? 0u
               : (uint32_t)(utils::get_first_bit_bigger(size) - (int)s->smallest_bits);

@@ -144,9 +144,22 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
     }
     utils::backoff();
   }
-  // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) in exact integer form
-  if (10ULL * r > nb && r >= 1 && 10ULL * (r - 1) < nb)
-    veb::insert(class_tree(s, t, h), seg);
+  // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) on the segment's fill: a
+  // return that brings its free blocks to at least ceil(nb / 10), without
+  // completing it, puts it back in its class tree when it is not there.  The
+  // reference compares the ring position mod nb instead ("TODO - make this
+  // actually calculate fill", memory_table.cuh:1181), which hides a segment
+  // emptied at ring position k until its ring wraps back to 10 %, about half
+  // of its blocks later.  The insert comes before this return's count
+  // increment, so the segment cannot be released in between.
+  {
+    const gal_tree& ct = class_tree(s, t, h);
+    if (!veb::query(ct, seg)) {
+      const int64_t f_after =
+          (int64_t)(int32_t)(atomic_load(&s->seg_state[seg]) >> 32) + 2;
+      if (10 * f_after >= (int64_t)nb && f_after < (int64_t)nb) veb::insert(ct, seg);
+    }
+  }
   const uint64_t old = atomic_add(&s->seg_state[seg], kActiveOne);
   const int32_t rid = (int32_t)(old >> 32);
   if ((int64_t)rid == (int64_t)nb - 2) {
@@ -672,8 +685,10 @@ __device__ __noinline__ uint64_t malloc_segments(gal_state* s, bool want, uint64
   return out;
 }
 
-// gallatin.cuh:1186-1276 malloc.
-__device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
+// gallatin.cuh:1186-1276 malloc.  Lanes with want = false take part in the
+// wave's cooperative work (tree searches) and get nullptr; every active lane
+// of the wave must call it.
+__device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size, bool want = true) {
   const bool det = deterministic(s);
   const uint32_t T = s->num_trees;
   const uint32_t lane = wave::lane();
@@ -681,9 +696,9 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size) {
   // size dispatch (gallatin.cuh:1141-1147, 1196-1258)
   const int cl = utils::get_first_bit_bigger(size < 1 ? 1 : size);
   uint32_t t = size < s->smallest ? 0u : (uint32_t)(cl - (int)s->smallest_bits);
-  uint32_t path = kPathSlice, cnt = 1;
+  uint32_t path = want ? kPathSlice : 0u, cnt = 1;
   uint64_t nseg = 0;
-  if (t >= T) {
+  if (want && t >= T) {
     const int bt = cl - (int)s->block_tree_bits;
     if (bt < 0) {
       cnt = 1u << (t - (T - 1));
@@ -815,6 +830,41 @@ __device__ inline uint64_t allocated_size(const gal_state* s, uint64_t bytes) {
 
 }  // namespace detail
 
+// Block counters under the reference's name (block.cuh:59-62: malloc_counter,
+// free_counter).
+using Block = gal_block;
+
+// The reference's table accessors that its tests call through
+// `allocator->table->` (tests/src/global_test.cu:105-107,
+// global_test_noprint.cu:125-157).  `table` is the instance itself seen
+// through this type, which adds no data.
+struct memory_table : gal_state {
+  // memory_table.cuh:1023-1027
+  __device__ uint64_t get_segment_from_ptr(void* allocation) const {
+    return ((uint64_t)allocation - (uint64_t)memory) >> seg_shift;
+  }
+  // memory_table.cuh:831-844: the chunk id, 0xFFFF for a free segment
+  __device__ uint16_t read_tree_id(uint64_t segment) const {
+    return (uint16_t)(utils::atomic_load(&seg_state[segment]) & 0xFFFFu);
+  }
+  // memory_table.cuh get_block_from_global_block_id: the counters of block id
+  __device__ Block* get_block_from_global_block_id(uint64_t global_block_id) const {
+    return blocks + block_position(global_block_id, block_bits);
+  }
+  // memory_table.cuh:1057-1061
+  __device__ uint64_t get_blocks_per_segment(uint16_t tree) const {
+    return blocks_per_segment >> tree;
+  }
+  // memory_table.cuh:1090-1127: slice offset of an allocation of class tree
+  __device__ uint64_t allocation_to_offset(void* allocation, uint16_t tree) const {
+    return detail::bytes_to_offset(this, (uint64_t)allocation - (uint64_t)memory, tree);
+  }
+  // memory_table.cuh:1346-1353
+  __device__ bool owns_allocation(void* allocation) const {
+    return ((uint64_t)allocation - (uint64_t)memory) < (num_segments << seg_shift);
+  }
+};
+
 // Drop-in for gallatin::allocators::Gallatin (gallatin.cuh:292-2035).
 template <uint64_t SegBytes, uint64_t Smallest, uint64_t Biggest>
 struct Gallatin : public gal_state {
@@ -896,6 +946,26 @@ struct Gallatin : public gal_state {
   // gallatin.cuh:2027 / memory_table.cuh:1346-1353
   __device__ bool owns_allocation(void* p) {
     return ((uint64_t)p - (uint64_t)memory) < (num_segments << seg_shift);
+  }
+  // gallatin.cuh:1842-1846
+  __device__ uint64_t allocation_to_offset(void* allocation, uint16_t tree_id) {
+    return table->allocation_to_offset(allocation, tree_id);
+  }
+  // gallatin.cuh:673-690: whether the allocation lies inside the pool, with
+  // the reference's diagnostic lines
+  __device__ bool check_alloc_valid(void* allocation) {
+    const uint64_t byte_difference = (uint64_t)allocation - (uint64_t)memory;
+    const uint64_t max_bytes = num_segments * bytes_per_segment;
+    printf("%llx inside of %llx\n", (unsigned long long)allocation,
+           (unsigned long long)memory);
+    if (byte_difference > max_bytes) {
+      printf("Byte difference: %llu > %llu\n", (unsigned long long)byte_difference,
+             (unsigned long long)max_bytes);
+      return false;
+    }
+    printf("Byte difference: %llu <= %llu\n", (unsigned long long)byte_difference,
+           (unsigned long long)max_bytes);
+    return true;
   }
 };
 

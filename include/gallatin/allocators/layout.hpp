@@ -127,6 +127,10 @@ enum gal_stat {
   kStatCount = 16
 };
 
+// The instance viewed through the reference's alloc_table accessors
+// (memory_table.cuh:229-1353); defined in gallatin.hpp.
+struct memory_table;
+
 struct gal_state {
   // geometry, fixed at creation
   uint64_t bytes_per_segment;
@@ -165,6 +169,9 @@ struct gal_state {
                                    // slots [p, p+1) * count / num_parts
   gal_tree seg_tree;
   gal_tree sub_trees[kMaxTrees * kMaxParts];  // class t, partition p at t*parts+p
+  // `table` of the reference's Gallatin (gallatin.cuh:306): the device
+  // address of this instance, viewed as a memory_table.
+  memory_table* table;
 };
 
 }  // namespace allocators

@@ -15,7 +15,7 @@
 // publishes the rest.  Requests that meet an exhausted range before the
 // crosser has published go through the per-wave path of gallatin.hpp, as do
 // requests the cache does not take (whole blocks, multi-segment runs, runs
-// of 4 KiB slices, the deterministic mode).  A range's unused slices stay
+// of 4 KiB slices).  A range's unused slices stay
 // claimed until cache_flush gives them back to their block at the end of
 // the kernel, so every slice is accounted for: after the flush the block
 // counters read as if each cached slice had been allocated and freed.
@@ -61,13 +61,14 @@ __device__ inline void cache_init(slice_cache& L) {
 // had.  The claim, the stale-block check and the block replacement follow
 // malloc_slices.
 __device__ inline uint64_t claim_range(gal_state* s, uint32_t t, uint32_t p, uint32_t want) {
+  const bool det = detail::deterministic(s);
   for (uint32_t attempts = 0;;) {
     uint64_t sb = kNullBlock;
     if (wave::is_first()) sb = detail::pinned_block(s, t, p);
     sb = wave::readfirst(sb);
     const uint32_t slot = (uint32_t)(sb >> 32);
     uint32_t blk = (uint32_t)sb;
-    if (blk == kNullBlock) blk = detail::refill_slot(s, t, p, slot, false);
+    if (blk == kNullBlock) blk = detail::refill_slot(s, t, p, slot, det);
     if (blk == kNullBlock) return 0;  // no block to be had
     if (blk != kRefilling) {
       uint32_t old = 0;
@@ -81,7 +82,7 @@ __device__ inline uint64_t claim_range(gal_state* s, uint32_t t, uint32_t p, uin
       // that keeps finding its class's block full
       if (tag_ok && ((take && cnt + take == kSlicesPerBlock) ||
                      (cnt >= kSlicesPerBlock && attempts >= kHelpAfter)))
-        detail::replace_block(s, t, p, slot, blk, false);
+        detail::replace_block(s, t, p, slot, blk, det);
       if (tag_ok && take) return ((uint64_t)blk << 32) | ((uint64_t)cnt << 16) | take;
     } else {
       utils::backoff();  // another wave refills the slot
@@ -101,7 +102,7 @@ __device__ inline void* cache_malloc(gal_state* s, slice_cache& L, bool want, ui
   const uint32_t lane = wave::lane();
   const int cl = utils::get_first_bit_bigger(size < 1 ? 1 : size);
   const uint32_t t = size < s->smallest ? 0u : (uint32_t)(cl - (int)s->smallest_bits);
-  const bool cached = want && t < T && !detail::deterministic(s);
+  const bool cached = want && t < T;
   const uint32_t p = detail::wave_part(s);
   uint64_t off = kFail;  // slice offset of this lane's allocation
   const uint64_t pend = __ballot(cached);

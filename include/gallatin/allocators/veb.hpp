@@ -224,7 +224,10 @@ __device__ inline uint64_t gather_multiple(const gal_tree& t, uint64_t n) {
         const uint64_t m = utils::low_mask(take) << bo;
         const uint64_t old = atomic_and(word(t, L, wi), ~m);
         if ((old & m) != m) {
-          atomic_or(word(t, L, wi), old & m);  // give back what we got here
+          // give back what we got here.  While our claim held the word at
+          // zero, a concurrent remove of another bit may have cleared the
+          // parent bit, so a give-back into an empty word re-syncs upward.
+          if (old & m && atomic_or(word(t, L, wi), old & m) == 0) sync_up(t, L, wi);
           ok = 0;
           break;
         }
@@ -239,7 +242,7 @@ __device__ inline uint64_t gather_multiple(const gal_tree& t, uint64_t n) {
           const uint64_t wi = rpos >> 6;
           const uint32_t bo = (uint32_t)(rpos & 63);
           const uint64_t take = (64 - bo) < rleft ? (64 - bo) : rleft;
-          atomic_or(word(t, L, wi), utils::low_mask(take) << bo);
+          if (atomic_or(word(t, L, wi), utils::low_mask(take) << bo) == 0) sync_up(t, L, wi);
           rpos += take;
           rleft -= take;
         }

@@ -360,26 +360,32 @@ int gal_queue_dequeue(gal_queue *q, uint64_t *out_dev, uint8_t *ok_dev, uint64_t
                       int serial, void *stream);
 
 /* ---- serving loop (no reference counterpart; on the global allocator) ----
- * Device-resident sessions, each owning a fixed_vector<uint64_t, 16, 8192>
- * (128 B .. 64 KiB) and a shared gal_table of kind 1.  A step (three
- * launches on `stream`): each live session ends with probability
- * end_q32 / 2^32 (vector freed) or appends tokens x 32 words, tokens drawn
- * from tokens_table[1024]; then `arrivals` new sessions are admitted if the
- * allocator has more than reserve_segments free segments (else refused);
- * then ht_ops table operations, 70 % insert / 30 % remove, keys uniform in
- * [1, 2^28].  Counters (device, 16 x u64): [0] admitted, [1] refused, [2]
- * ended, [3] words appended, [4] words refused (vector full), [5] chunks
- * allocated, [6] chunk allocations failed, [7] chunks freed, [8] table
- * inserts, [9] removes that found the key, [10] removes of absent keys,
- * [11] inserts refused. */
+ * Device-resident sessions, each owning a chunked_vector<uint64_t, 16,
+ * 8192, 512> (include/gallatin/data_structs/chunked_vector.hpp: chunks of
+ * 128 B doubling to 64 KiB, then up to 512 more 64 KiB chunks) and a shared
+ * gal_table of kind 1.  A step (three launches on `stream`): each live
+ * session ends with probability end_q32 / 2^32 (vector freed) or appends
+ * tokens x 32 words, tokens drawn from tokens_table[1024]; then `arrivals`
+ * new sessions are admitted if the allocator has more than
+ * reserve_segments free segments (else refused); then ht_ops table
+ * operations, 70 % insert / 30 % remove, keys uniform in [1, 2^28].
+ * Counters (device, 16 x u64): [0] admitted, [1] refused, [2] ended, [3]
+ * words appended, [4] words refused (a chunk could not be allocated, or
+ * the vector is at its 4,210,688-word limit), [5] chunks allocated, [6]
+ * chunk allocations failed, [7] chunks freed, [8] table inserts, [9]
+ * removes that found the key, [10] removes of absent keys, [11] inserts
+ * refused. */
 typedef struct gal_serving gal_serving;
 int gal_serving_create(uint32_t max_sessions, uint64_t seed,
                        const uint32_t *tokens_table_host, gal_serving **out);
 int gal_serving_step(gal_serving *s, uint32_t step, uint32_t arrivals,
                      uint64_t reserve_segments, uint32_t end_q32, uint32_t ht_ops,
                      uint64_t *counters_dev, void *stream);
-/* Live sessions, words checked, words not equal to (id << 32) | position. */
-int gal_serving_check(gal_serving *s, uint64_t *live, uint64_t *words, uint64_t *bad);
+/* Live sessions, words checked, words not equal to (id << 32) | position,
+ * and the bytes of the chunks (and chunk indexes) the sessions hold.
+ * Synchronous. */
+int gal_serving_check(gal_serving *s, uint64_t *live, uint64_t *words, uint64_t *bad,
+                      uint64_t *held_bytes);
 /* Keys, buckets and refused inserts of the session table. */
 int gal_serving_table(gal_serving *s, uint64_t *keys, uint64_t *buckets, uint64_t *failed);
 /* Frees every live session's vector and the table. */
@@ -410,14 +416,30 @@ int gal_run_batches(gal_allocator *a, const gal_batch_op *ops_dev,
                     uint64_t nslots, uint64_t *counters_dev, uint32_t threads,
                     uint32_t batched, uint32_t workgroups, void *stream);
 
+/* Validity of a stream on the host (ops in host memory).  Every op has
+ * kind 0 or 1 and a slot below nslots; a slot is filled again only after
+ * its pointer was freed, and a free names a slot whose malloc ran in an
+ * earlier batch.  persistent != 0 adds the requirement of the plan below:
+ * each slot is filled at most once in the stream.  GAL_ERR_INVALID names
+ * the first op that breaks a rule (gal_last_error).  No reference
+ * counterpart. */
+int gal_check_batch_stream(const gal_batch_op *ops_host, const uint64_t *offsets,
+                           uint32_t nbatches, uint64_t nslots, int persistent);
+
 /* The stream as one persistent launch, captured once into a hipGraph (on a
  * stream of its own: a reset kernel, the persistent kernel, the cleanup)
  * and replayed with gal_batch_plan_launch.  Workgroups take chunks of at
  * most 4 x threads ops (never across batches) in stream order from a
  * device queue and serve a chunk's mallocs before its frees; a free waits
  * only for its own table entry (bounded: a wait that runs out counts in
- * counter [5]), so each slot is filled at most once per stream.  The
- * buffers must stay valid until the plan is destroyed. */
+ * counter [5]).
+ * REQUIREMENT: each slot is filled by at most one malloc in the whole
+ * stream, and every free names a slot filled in an earlier batch.  The
+ * chunks run at once, so a slot filled twice could be overwritten before
+ * its first pointer is freed.  gal_batch_plan_create copies the ops to the
+ * host once and refuses a stream that breaks this (gal_check_batch_stream
+ * with persistent = 1) with GAL_ERR_INVALID.  The buffers must stay valid
+ * until the plan is destroyed. */
 typedef struct gal_batch_plan gal_batch_plan;
 int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
                           const uint64_t *offsets, uint32_t nbatches,

@@ -474,10 +474,18 @@ static void return_block(orc_alloc *a, uint32_t blk, uint64_t seg, int t) {
   uint32_t r = (uint32_t)(pos % nb);
   a->ring[seg * a->B + r] = blk;
   a->q_final[seg] = pos + 1; /* finalize_free_queue */
-  /* REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) in exact integer form:
-   * r/nb >= 0.1 and (uint)(r-1)/nb < 0.1 */
-  if (10ULL * r > nb && r >= 1 && 10ULL * (r - 1) < nb)
-    orc_veb_insert(a->sub_trees[t], seg);
+  (void)r;
+  /* REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) on the segment's fill (fix
+   * F7): a return that brings the free blocks (active + 1) to at least
+   * ceil(nb / 10), without completing the segment, inserts it into its class
+   * tree when it is not there.  The reference compares the ring position
+   * r = pos % nb with the cutoff instead (memory_table.cuh:1175-1182, "TODO
+   * - make this actually calculate fill"). */
+  if (!orc_veb_query(a->sub_trees[t], seg)) {
+    int64_t f_after = (int64_t)a->active[seg] + 2;
+    if (10 * f_after >= (int64_t)nb && f_after < (int64_t)nb)
+      orc_veb_insert(a->sub_trees[t], seg);
+  }
   int32_t rid = a->active[seg]++;
   int dereg = 0;
   if ((int64_t)rid == (int64_t)nb - 2) {

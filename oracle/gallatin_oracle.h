@@ -43,6 +43,15 @@
  *      fails once no block can be had and a multi-segment request fails
  *      when no free run of n segments exists, instead of repeating the same
  *      search 500 times (gallatin.cuh:1226-1251).
+ *   F7 the 10 % re-register of a segment into its class tree follows the
+ *      segment's fill: a block return that brings its free blocks to at
+ *      least ceil(nb / 10) without completing it inserts the segment when
+ *      it is not in the tree.  The reference tests the ring position
+ *      pos % nb of the returned block instead (gallatin.cuh:1646 with
+ *      memory_table.cuh:1175-1182, marked "TODO - make this actually
+ *      calculate fill"), so a segment whose last block was handed out at
+ *      ring position k becomes findable again only when its ring wraps to
+ *      10 %: on average half of its blocks free and unusable.
  */
 #ifndef GALLATIN_ORACLE_H
 #define GALLATIN_ORACLE_H

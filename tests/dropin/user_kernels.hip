@@ -101,6 +101,91 @@ extern "C" int user_run(uint64_t pool_bytes, uint64_t n, uint64_t size,
 
 namespace {
 
+// tests/src/global_test_noprint.cu:81-160 with its checks counted instead of
+// printed: each thread mallocs, writes its id, and the check kernel finds
+// the allocation's segment, tree and block through the table accessors and
+// counts allocations per block.
+__global__ void table_malloc(uint64_t n, uint64_t size, uint64_t** bitarray) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t* malloc = static_cast<uint64_t*>(global_malloc(size));
+  bitarray[tid] = malloc;
+  if (malloc) malloc[0] = tid;
+}
+
+__global__ void table_check(uint64_t n, uint64_t size, uint64_t** bitarray,
+                            uint64_t* block_counts, unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t* malloc = bitarray[tid];
+  if (malloc == nullptr) {
+    atomicAdd(&counts[0], 1ULL);
+    return;
+  }
+  if (tid == 0 && !global_gallatin->check_alloc_valid((void*)malloc)) atomicAdd(&counts[1], 1ULL);
+  uint64_t segment = global_gallatin->table->get_segment_from_ptr((void*)malloc);
+  uint16_t tree_id = global_gallatin->table->read_tree_id(segment);
+  uint64_t offset = global_gallatin->allocation_to_offset((void*)malloc, tree_id);
+  uint64_t block_id = offset / 4096;
+  atomicAdd((unsigned long long*)&block_counts[block_id], 1ULL);
+  // the accessors agree with the geometry: the pointer lies in its segment,
+  // the segment belongs to the size's class, the block is that class's
+  const uint64_t rel = (uint64_t)((char*)malloc - global_gallatin->memory);
+  const uint16_t cls = (uint16_t)(global_gallatin->get_first_bit_bigger(size < 16 ? 16 : size) - 4);
+  Block* block_ptr = global_gallatin->table->get_block_from_global_block_id(block_id);
+  if ((rel >> 24) != segment || tree_id != cls || block_id / 256 != segment ||
+      (block_ptr->malloc_counter >> 20) != tree_id)
+    atomicAdd(&counts[1], 1ULL);
+  if (malloc[0] != tid) atomicAdd(&counts[2], 1ULL);
+}
+
+// check_blocks (global_test_noprint.cu:165-185): no block holds more than
+// 4096 allocations.
+__global__ void table_check_blocks(const uint64_t* blocks, uint64_t nblocks,
+                                   unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid < nblocks && blocks[tid] > 4096) atomicAdd(&counts[3], 1ULL);
+}
+
+__global__ void table_free(uint64_t n, uint64_t** bitarray) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid < n && bitarray[tid]) global_free(bitarray[tid]);
+}
+
+}  // namespace
+
+// counts_out: [0] misses, [1] accessor disagreements, [2] ids not read
+// back, [3] blocks counted past 4096.
+extern "C" int user_table_accessors(uint64_t pool_bytes, uint64_t n, uint64_t size,
+                                    unsigned long long* counts_out) {
+  init_global_allocator(pool_bytes, 42, false);
+  const uint64_t nblocks = (pool_bytes >> 24) * 256;
+  uint64_t** bitarray = nullptr;
+  uint64_t* block_counts = nullptr;
+  unsigned long long* counts = nullptr;
+  if (hipMalloc(&bitarray, n * sizeof(uint64_t*)) != hipSuccess ||
+      hipMalloc(&block_counts, nblocks * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&counts, 4 * sizeof(unsigned long long)) != hipSuccess)
+    return -1;
+  (void)hipMemset(block_counts, 0, nblocks * sizeof(uint64_t));
+  (void)hipMemset(counts, 0, 4 * sizeof(unsigned long long));
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  table_malloc<<<blocks, 256>>>(n, size, bitarray);
+  table_check<<<blocks, 256>>>(n, size, bitarray, block_counts, counts);
+  table_check_blocks<<<(unsigned)((nblocks + 255) / 256), 256>>>(block_counts, nblocks, counts);
+  table_free<<<blocks, 256>>>(n, bitarray);
+  if (hipDeviceSynchronize() != hipSuccess) return -2;
+  free_global_allocator();
+  const hipError_t e = hipMemcpy(counts_out, counts, 4 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost);
+  (void)hipFree(bitarray);
+  (void)hipFree(block_counts);
+  (void)hipFree(counts);
+  return e == hipSuccess ? 0 : -2;
+}
+
+namespace {
+
 using user_vector = fixed_vector<uint64_t, 16, 1ULL << 20>;
 using user_table = extendible_hash_table<uint64_t, 0ULL, ~0ULL, uint64_t, 15, 10, 20>;
 using user_chain = chaining_table<uint64_t, uint64_t, 4>;

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from gallatin_amd import batch_streams as B
+from gallatin_amd.batch_streams import zipf_stream
 
 
 def test_bounded_zipf_range_and_head():
@@ -45,3 +46,41 @@ def test_check_stream_catches_a_double_free():
     bad["slot"][frees[-1]] = bad["slot"][frees[0]]   # free the same slot twice
     with pytest.raises(AssertionError):
         B.check_stream(bad, off, nslots)
+
+
+def _ops(rows):
+    import gallatin_amd as G
+    a = np.zeros(len(rows), dtype=G.BATCH_OP_DTYPE)
+    for i, (kind, slot) in enumerate(rows):
+        a[i] = (16, kind, slot)
+    return a
+
+
+def test_stream_check_accepts_the_zipf_stream():
+    import gallatin_amd as G
+    ops, off, nslots = zipf_stream(5, nbatches=32, max_count=1 << 10)
+    G.check_batch_stream(ops, off, nslots, persistent=True)
+
+
+@pytest.mark.parametrize("rows,offsets,persistent,msg", [
+    ([(0, 0), (0, 0)], [0, 1, 2], True, "filled a second time"),
+    ([(0, 0), (1, 0), (0, 0)], [0, 1, 2, 3], True, "filled a second time"),
+    ([(0, 0), (0, 0)], [0, 1, 2], False, "before its pointer was freed"),
+    ([(0, 0), (1, 0)], [0, 2], False, "without a malloc in an earlier batch"),
+    ([(1, 3)], [0, 1], False, "without a malloc in an earlier batch"),
+    ([(0, 9)], [0, 1], False, "slot beyond nslots"),
+])
+def test_stream_check_refuses(rows, offsets, persistent, msg):
+    """The plan's slot rules are checked on the host: a slot filled twice in
+    a persistent stream, filled again before its free, freed in its own
+    batch or never filled, or out of range is refused with its op named."""
+    import gallatin_amd as G
+    with pytest.raises(G.GallatinError, match=msg):
+        G.check_batch_stream(_ops(rows), offsets, 4, persistent=persistent)
+
+
+def test_stream_check_allows_reuse_across_batches_eagerly():
+    """Per-batch launches may refill a slot after its free in a later batch."""
+    import gallatin_amd as G
+    G.check_batch_stream(_ops([(0, 0), (1, 0), (0, 0), (1, 0)]), [0, 1, 2, 3, 4], 4,
+                         persistent=False)

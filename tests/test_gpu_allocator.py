@@ -85,6 +85,45 @@ def test_wave_known_answer(gpu):
     a.free_on_device()
 
 
+def test_size_boundary_known_answers(gpu):
+    """32768 / 32769 B and exactly 16 MiB: the last slice run, the first
+    whole block and the largest class's block."""
+    seq = G["size_boundaries_1GiB"]
+    a = make(gpu, 1 << 30)
+    res = a.run_wave_stream(_single_lane_steps([s["size"] for s in seq]))
+    assert [int(res[i * 64]) for i in range(len(seq))] == [s["offset"] for s in seq]
+    o = oracle.Allocator(1 << 30, limits=QUICK)
+    o.run_stream(_single_lane_steps([s["size"] for s in seq]))
+    compare_metadata(a, o)
+    a.free_on_device()
+
+
+def test_ring_reuse_and_reregister_known_answer(gpu):
+    """Blocks come back through the segment ring first in, first out, and a
+    segment rejoins its class tree at the return that brings its free
+    blocks to 10 % (fix F7); metadata equals the oracle's afterwards."""
+    seq = G["ring_reuse_and_reregister_1GiB"]
+    steps = np.zeros(len(seq), H.STEP_DTYPE)
+    for i, op in enumerate(seq):
+        steps[i]["mask"] = 1
+        if op["op"] == "malloc":
+            steps[i]["kind"] = 0
+            steps[i]["arg"][0] = op["size"]
+        else:
+            steps[i]["kind"] = 1
+            steps[i]["arg"][0] = op["of"] * 64
+    a = make(gpu, 1 << 30)
+    res = a.run_wave_stream(steps)
+    for i, op in enumerate(seq):
+        if op["op"] == "malloc":
+            assert int(res[i * 64]) == op["offset"], (i, op)
+    o = oracle.Allocator(1 << 30, limits=QUICK)
+    assert np.array_equal(o.run_stream(steps), res)
+    compare_metadata(a, o)
+    assert a.info()["errors"] == 0
+    a.free_on_device()
+
+
 def test_multi_segment_known_answers(gpu):
     seq = G["multi_segment_4GiB"]
     a = make(gpu, 4 << 30)

@@ -52,3 +52,20 @@ def test_user_kernels_data_structures():
     rc = _lib().user_structures(1 << 16, counts)
     assert rc == 0
     assert list(counts) == [0] * 5, "vector, table, chain, queue, block queue errors"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [16, 100, 4096])
+def test_user_kernels_table_accessors(size):
+    """The reference's mainline test body (tests/src/global_test_noprint.cu:
+    81-185) compiles against the headers unchanged apart from the include
+    line: it reaches segment, tree id, slice offset and block counters
+    through global_gallatin->table and allocation_to_offset, and those agree
+    with the geometry; ids read back and no block counts past 4096."""
+    import torch
+    torch.cuda.set_device(0)
+    L = _lib()
+    L.user_table_accessors.argtypes = [ctypes.c_uint64] * 3 + [ctypes.POINTER(ctypes.c_ulonglong)]
+    counts = (ctypes.c_ulonglong * 4)()
+    assert L.user_table_accessors(1 << 30, 1 << 16, size, counts) == 0
+    assert list(counts) == [0, 0, 0, 0], "misses, accessor, id, block-count errors"

@@ -349,3 +349,29 @@ def test_churn_sizes_match_streams_header():
     # log-uniform: about 1/12 of draws per octave
     octs = np.bincount(np.floor(np.log2(s.astype(np.float64))).astype(int), minlength=12)
     assert np.all(np.abs(octs / len(s) - 1 / 12) < 0.01)
+
+
+def test_size_boundary_known_answers():
+    """32768 B is the last run of 4 KiB slices, 32769 B the first whole
+    block; exactly 16 MiB is a whole block of the largest class."""
+    a = oracle.Allocator(1 << 30)
+    for step in G["size_boundaries_1GiB"]:
+        assert a.malloc(step["size"]) == step["offset"], step
+    assert a.errors == 0
+
+
+def test_ring_reuse_and_reregister_known_answer():
+    """Blocks come back through the segment ring first in, first out, and a
+    segment rejoins its class tree at the return that brings its free
+    blocks to 10 % (4 of 32)."""
+    a = oracle.Allocator(1 << 30)
+    got = []
+    for op in G["ring_reuse_and_reregister_1GiB"]:
+        if op["op"] == "malloc":
+            r = a.malloc(op["size"])
+            assert r == op["offset"], (len(got), op)
+            got.append(r)
+        else:
+            a.free(got[op["of"]])
+            got.append(None)
+    assert a.errors == 0

@@ -54,79 +54,56 @@ def test_sizes_cover_their_ranges():
 
 
 def test_glibc_baseline_frees_every_request():
-    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
-
-    class R(ctypes.Structure):
-        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
-                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
-                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
-                    ("pad", ctypes.c_uint32)]
-    lib.gal_cpu_big_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
-                                      ctypes.POINTER(R)]
-    r = R()
-    assert lib.gal_cpu_big_glibc(3, 0, 3, 509, 3, 4, ctypes.byref(r)) == 0
-    assert r.misses == 0 and r.bad == 0
-    assert r.pairs == 3 * 512 and r.bytes == 2 * 8 * 3 * 512 and r.threads == 4
-
-
-def _cpu_lib():
-    lib = ctypes.CDLL(os.path.join(ROOT, "gallatin_amd", "libgallatin_cpu.so"))
-
-    class R(ctypes.Structure):
-        _fields_ = [("pairs", ctypes.c_uint64), ("misses", ctypes.c_uint64),
-                    ("bad", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
-                    ("seconds", ctypes.c_double), ("threads", ctypes.c_uint32),
-                    ("pad", ctypes.c_uint32)]
-    return lib, R
+    from gallatin_amd import cpu_baseline as C
+    r = C.big(3, 0, 3, 509, 3, 4)
+    assert r["misses"] == 0 and r["bad"] == 0
+    assert r["pairs"] == 3 * 512 and r["bytes"] == 2 * 8 * 3 * 512 and r["threads"] == 4
 
 
 def test_glibc_plain_baseline():
-    lib, R = _cpu_lib()
-    lib.gal_cpu_plain_glibc.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
-                                        ctypes.POINTER(R)]
-    r = R()
-    assert lib.gal_cpu_plain_glibc(10000, 3, 4, ctypes.byref(r)) == 0
-    assert r.pairs == 30000 and r.misses == 0 and r.bad == 0 and r.threads == 4
+    from gallatin_amd import cpu_baseline as C
+    r = C.plain(10000, 3, 4)
+    assert r["pairs"] == 30000 and r["misses"] == 0 and r["bad"] == 0 and r["threads"] == 4
 
 
 def test_glibc_batches_baseline():
     """Every malloc of a small Zipf stream is freed exactly once (in a later
     batch or at the end) with its tag intact."""
     from gallatin_amd import batch_streams as B
+    from gallatin_amd import cpu_baseline as C
     ops, off, nslots = B.zipf_stream(11, nbatches=48, max_count=1 << 13)
     mallocs, _ = B.check_stream(ops, off, nslots)
-    ops = np.ascontiguousarray(ops)
-    off = np.ascontiguousarray(off, dtype=np.uint64)
-    lib, R = _cpu_lib()
-    lib.gal_cpu_batches_glibc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
-                                          ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(R)]
-    r = R()
-    assert lib.gal_cpu_batches_glibc(ops.ctypes.data, off.ctypes.data, len(off) - 1, nslots, 4,
-                                     ctypes.byref(r)) == 0
-    assert r.misses == 0 and r.bad == 0
-    assert r.pairs == mallocs
+    r = C.batches(ops, off, nslots, 4)
+    assert r["misses"] == 0 and r["bad"] == 0
+    assert r["pairs"] == mallocs
 
 
 def test_glibc_serving_baseline():
     """A small serving run on host threads: sessions live, append and end,
-    the table holds keys, and every ending session's first word is intact."""
+    the table holds keys, and every ending session's first word is intact.
+    Sessions grow past 128 KiB (the doubling chunks) into 64 KiB chunks."""
+    from gallatin_amd import cpu_baseline as C
     from gallatin_amd import serving as S
+    r, ms = C.serving(5, 10, 3, 4096, 200.0, 0.05, 4096, S.tokens_table(), 1 << 40, 0.05, 4)
+    assert r["bad"] == 0 and r["live"] > 0 and r["ended"] > 0 and r["table_keys"] > 0
+    assert r["appended_words"] > 0 and r["table_ops"] == 13 * 4096 and np.all(ms > 0)
+    assert r["refused"] == 0 and r["admitted"] > 0
+    # 512 slots filled in about 10 steps, then 20 more steps of about 1056
+    # words each: most sessions are past the 16384 words of the doubling
+    # chunks and hold 64 KiB chunks beyond them
+    r, _ = C.serving(7, 0, 30, 512, 50.0, 0.01, 0, S.tokens_table(), 1 << 40, 0.05, 4)
+    assert r["bad"] == 0 and r["live"] > 400
+    assert r["held_bytes"] > r["live"] * 16384 * 8 // 2
 
-    class SR(ctypes.Structure):
-        _fields_ = [("appended_words", ctypes.c_uint64), ("ended", ctypes.c_uint64),
-                    ("table_ops", ctypes.c_uint64), ("table_keys", ctypes.c_uint64),
-                    ("bad", ctypes.c_uint64), ("live", ctypes.c_uint32),
-                    ("threads", ctypes.c_uint32)]
-    lib, _ = _cpu_lib()
-    lib.gal_cpu_serving_glibc.argtypes = [
-        ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
-        ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
-        ctypes.POINTER(SR)]
-    tok = np.ascontiguousarray(S.tokens_table(), dtype=np.uint32)
-    ms = np.zeros(3)
-    r = SR()
-    assert lib.gal_cpu_serving_glibc(5, 10, 3, 4096, 200.0, 0.05, 4096, tok.ctypes.data, 4,
-                                     ms.ctypes.data, ctypes.byref(r)) == 0
-    assert r.bad == 0 and r.live > 0 and r.ended > 0 and r.table_keys > 0
-    assert r.appended_words > 0 and r.table_ops == 13 * 4096 and np.all(ms > 0)
+
+def test_glibc_serving_gate():
+    """The host gate refuses arrivals once the chunk bytes held reach the
+    pool budget less its reserve, and admits none when the budget is 0."""
+    from gallatin_amd import cpu_baseline as C
+    from gallatin_amd import serving as S
+    r, _ = C.serving(5, 0, 8, 4096, 200.0, 0.05, 0, S.tokens_table(), 0, 0.05, 4)
+    assert r["admitted"] == 0 and r["refused"] > 0 and r["live"] == 0
+    # a budget of about 20 sessions' first steps: admission stops early
+    r, _ = C.serving(5, 2, 10, 4096, 200.0, 0.01, 0, S.tokens_table(), 24 << 20, 0.05, 4)
+    assert r["refused"] > 0 and r["bad"] == 0
+    assert r["held_bytes"] < (24 << 20) * 4
