@@ -66,11 +66,16 @@ def parse():
     p.add_argument("--big-steps", type=int, default=20)
     p.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     p.add_argument("--serving-pool-gib", type=int, default=200)
-    p.add_argument("--serving-steps", type=int, default=200)
-    p.add_argument("--serving-warmup", type=int, default=600)
-    p.add_argument("--serving-reserve", type=float, default=0.05,
+    # the timed window covers a whole admission cycle: the pool fills, the
+    # gate closes, long-lived sessions drain, the gate reopens
+    p.add_argument("--serving-steps", type=int, default=1000)
+    p.add_argument("--serving-warmup", type=int, default=300)
+    p.add_argument("--serving-reserve", type=float, default=0.10,
                    help="admission reserve: fraction of the pool's segments kept free")
     p.add_argument("--cpu-serving-steps", type=int, default=30)
+    p.add_argument("--serving-gate", type=int, default=0, choices=(0, 1),
+                   help="admission gate: 0 free segments of the segment tree, 1 free memory "
+                        "in segments (free segments plus free blocks of formatted segments)")
     p.add_argument("--detail-json", default="",
                    help="also write the full result (every section, the autotune table) here")
     return p.parse_args()
@@ -339,7 +344,8 @@ def serving_run(args, G, torch):
     G.init_global_allocator(args.serving_pool_gib << 30, args.seed, print_info=False)
     try:
         r = serving.run(torch, steps=args.serving_steps, warmup=args.serving_warmup,
-                        seed=args.seed, reserve_fraction=args.serving_reserve, log=log)
+                        seed=args.seed, reserve_fraction=args.serving_reserve, log=log,
+                        gate=args.serving_gate)
         g = G.global_allocator()
         r["allocator_errors"] = g.info()["errors"]
     finally:
@@ -347,7 +353,9 @@ def serving_run(args, G, torch):
     r["workload"] = (f"{args.serving_pool_gib} GiB pool, Poisson(2000) arrivals/step, "
                      "geometric(0.01) lifetime, lognormal(3,1) tokens x 256 B per session-step "
                      "into chunks of 128 B doubling to 64 KiB then 64 KiB each, "
-                     f"{args.serving_reserve:.0%} free-segment admission reserve, 2^20 table "
+                     f"{args.serving_reserve:.0%} admission reserve of "
+                     f"{'free memory in segments' if args.serving_gate else 'free segments'}, "
+                     "2^20 table "
                      "ops/step (70% insert / 30% remove, keys in [1, 2^28]), "
                      f"{args.serving_warmup} warm-up steps")
     r["pool_bytes"] = args.serving_pool_gib << 30
@@ -588,7 +596,8 @@ def compact(out):
     if sv:
         cpu = out.get("cpu_glibc_serving") or {}
         w["serving"] = dict(
-            _pick(sv, "p50_step_ms", "p99_step_ms", "mean_step_ms", "table_ops_per_s",
+            _pick(sv, "p50_step_ms", "p99_step_ms", "mean_step_ms", "p50_head_ms", "p99_head_ms",
+                  "steps", "warmup", "table_ops_per_s",
                   "words_per_s", "allocator_ops_per_s", "live_sessions", "session_words_bad",
                   "table_keys", "refused_sessions", "words_refused", "chunk_failures",
                   "allocator_errors", "gb_live"),

@@ -53,7 +53,7 @@ CAPI_SYMBOLS = [
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
     "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
     "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
-    "gal_serving_check", "gal_serving_table", "gal_serving_destroy",
+    "gal_serving_check", "gal_serving_table", "gal_serving_destroy", "gal_serving_gate",
     "gal_queue_create", "gal_queue_destroy", "gal_queue_enqueue", "gal_queue_dequeue",
     "gal_chain_create", "gal_chain_destroy", "gal_chain_insert", "gal_chain_query",
     "gal_chain_stats", "gal_chain_lengths", "gal_chain_print",
@@ -154,7 +154,8 @@ def lib():
         L.gal_table_remove.argtypes = [_p, _p, _u64, _p, _p]
         L.gal_table_stats.argtypes = [_p, _p, _p, _p, _p]
         L.gal_serving_create.argtypes = [_u32, _u64, _p, ctypes.POINTER(_p)]
-        L.gal_serving_step.argtypes = [_p, _u32, _u32, _u64, _u32, _u32, _p, _p]
+        L.gal_serving_step.argtypes = [_p, _u32, _u32, _u64, _u32, _u32, _u32, _p, _p]
+        L.gal_serving_gate.argtypes = [_p, _p, _p]
         L.gal_serving_check.argtypes = [_p, _p, _p, _p, _p]
         L.gal_serving_table.argtypes = [_p, _p, _p, _p]
         L.gal_serving_destroy.argtypes = [_p]

@@ -930,33 +930,56 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
   }
 }
 
-// Free segments of the global allocator (bits set in the segment tree).
+// Free memory of the global allocator for the admission gate (one
+// workgroup).  out[0]: free segments (bits set in the segment tree); out[1]:
+// the free blocks of formatted segments, each weighted by its share of a
+// segment, times blocks_per_segment (a block of class t holds 2^t / B of a
+// segment), so out[0] + out[1] / B is the free memory in segments.
 __global__ void k_serve_free_segments(uint64_t* out) {
-  const gallatin::allocators::gal_tree& t = global_gallatin->seg_tree;
+  const gal_state* g = global_gallatin;
+  const gallatin::allocators::gal_tree& t = g->seg_tree;
   const uint32_t L = t.num_layers - 1;
-  uint64_t c = 0;
-  for (uint64_t w = threadIdx.x; w < t.layer_words[L]; w += blockDim.x)
-    c += __popcll(gallatin::utils::atomic_load(t.words + t.layer_off[L] + w));
-  for (int off = 32; off >= 1; off >>= 1) c += gallatin::wave::fetch(c, (int)(gallatin::wave::lane() ^ (uint32_t)off));
-  if (gallatin::wave::lane() == 0) atomicAdd((unsigned long long*)out, (unsigned long long)c);
+  uint64_t c = 0, w = 0;
+  for (uint64_t i = threadIdx.x; i < t.layer_words[L]; i += blockDim.x)
+    c += __popcll(gallatin::utils::atomic_load(t.words + t.layer_off[L] + i));
+  for (uint64_t seg = threadIdx.x; seg < g->num_segments; seg += blockDim.x) {
+    const uint64_t st = gallatin::utils::atomic_load(&g->seg_state[seg]);
+    const uint32_t tree = (uint32_t)(st & 0xFFFF);
+    const int32_t active = (int32_t)(st >> 32);
+    if (tree < g->num_trees && active >= 0) w += (uint64_t)(active + 1) << tree;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    c += gallatin::wave::fetch(c, (int)(gallatin::wave::lane() ^ (uint32_t)off));
+    w += gallatin::wave::fetch(w, (int)(gallatin::wave::lane() ^ (uint32_t)off));
+  }
+  if (gallatin::wave::lane() == 0) {
+    atomicAdd((unsigned long long*)&out[0], (unsigned long long)c);
+    atomicAdd((unsigned long long*)&out[1], (unsigned long long)w);
+  }
 }
 
 __global__ void k_serve_reset(uint32_t* queue, uint64_t* free_segments) {
   if (threadIdx.x == 0) {
     *queue = 0;
-    *free_segments = 0;
+    free_segments[0] = 0;
+    free_segments[1] = 0;
   }
 }
 
+// gate 0: admit while the segment tree holds more than `reserve` free
+// segments; gate 1: while the free memory, whole free segments plus the free
+// blocks of formatted segments counted in segments, exceeds `reserve`.
 __global__ void k_serve_admit(serving_session* ss, uint32_t* free_slots, int32_t* top,
-                              uint32_t arrivals, uint64_t reserve,
+                              uint32_t arrivals, uint64_t reserve, uint32_t gate,
                               const uint64_t* free_segments, uint32_t step, uint64_t* ctr) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= arrivals) return;
   // One pop of the free-slot stack per wave: the wave takes as many slots
   // as it has arrivals, and the lanes past what the stack held give theirs
   // back.
-  const bool room = *free_segments > reserve;
+  const uint64_t free_now =
+      free_segments[0] + (gate ? free_segments[1] / global_gallatin->blocks_per_segment : 0);
+  const bool room = free_now > reserve;
   const uint64_t want = __ballot(room);
   const uint32_t rank = gallatin::wave::count(want & gallatin::wave::lanes_below());
   int32_t top0 = 0;
@@ -1061,7 +1084,7 @@ int gal_serving_create(uint32_t max_sessions, uint64_t seed, const uint32_t* tok
       (e = hipMalloc(&sv->free_slots, sizeof(uint32_t) * max_sessions)) != hipSuccess ||
       (e = hipMalloc(&sv->free_top, sizeof(int32_t))) != hipSuccess ||
       (e = hipMalloc(&sv->queue, sizeof(uint32_t))) != hipSuccess ||
-      (e = hipMalloc(&sv->free_segments, sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&sv->free_segments, 2 * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMalloc(&sv->tokens, sizeof(uint32_t) * 1024)) != hipSuccess ||
       (e = hipMemcpy(sv->tokens, tokens_table, sizeof(uint32_t) * 1024,
                      hipMemcpyHostToDevice)) != hipSuccess) {
@@ -1086,8 +1109,10 @@ int gal_serving_create(uint32_t max_sessions, uint64_t seed, const uint32_t* tok
 }
 
 int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t reserve_segments,
-                     uint32_t end_q32, uint32_t ht_ops, uint64_t* counters_dev, void* stream) {
+                     uint32_t gate, uint32_t end_q32, uint32_t ht_ops, uint64_t* counters_dev,
+                     void* stream) {
   if (!sv || !counters_dev) return fail(GAL_ERR_INVALID, "null argument");
+  if (gate > 1) return fail(GAL_ERR_INVALID, "gate is 0 (free segments) or 1 (free memory)");
   if (arrivals >= (1u << 20)) return fail(GAL_ERR_INVALID, "at most 2^20 arrivals per step");
   hipStream_t st = (hipStream_t)stream;
   int dev = 0, cus = 256;
@@ -1100,11 +1125,12 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
   k_serve_sessions<<<groups, 256, 0, st>>>(sv->sessions, sv->max_sessions, sv->free_slots,
                                            sv->free_top, sv->queue, sv->tokens, sv->seed, step,
                                            end_q32, counters_dev);
-  k_serve_free_segments<<<1, 256, 0, st>>>(sv->free_segments);
+  k_serve_free_segments<<<1, 1024, 0, st>>>(sv->free_segments);
   if (arrivals)
     k_serve_admit<<<(arrivals + 255) / 256, 256, 0, st>>>(sv->sessions, sv->free_slots,
                                                           sv->free_top, arrivals, reserve_segments,
-                                                          sv->free_segments, step, counters_dev);
+                                                          gate, sv->free_segments, step,
+                                                          counters_dev);
   if (ht_ops)
     k_serve_table<<<(ht_ops + 255) / 256, 256, 0, st>>>(sv->table, sv->seed, step, ht_ops,
                                                         counters_dev);
@@ -1133,6 +1159,19 @@ int gal_serving_check(gal_serving* sv, uint64_t* live, uint64_t* words, uint64_t
   if (live) *live = h[1];
   if (words) *words = h[2];
   if (held_bytes) *held_bytes = h[3];
+  return GAL_OK;
+}
+
+int gal_serving_gate(gal_serving* sv, uint64_t* free_segments, uint64_t* free_memory_segments) {
+  if (!sv) return fail(GAL_ERR_INVALID, "null argument");
+  uint64_t h[2] = {0, 0};
+  const hipError_t e = hipMemcpy(h, sv->free_segments, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "gate copy");
+  gal_allocator* g = gal_global_handle();
+  gal_info info;
+  if (!g || gal_get_info(g, &info) != GAL_OK) return fail(GAL_ERR_INVALID, "no global allocator");
+  if (free_segments) *free_segments = h[0];
+  if (free_memory_segments) *free_memory_segments = h[0] + h[1] / info.blocks_per_segment;
   return GAL_OK;
 }
 

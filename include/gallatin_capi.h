@@ -369,6 +369,10 @@ int gal_queue_dequeue(gal_queue *q, uint64_t *out_dev, uint8_t *ok_dev, uint64_t
  * new sessions are admitted if the allocator has more than
  * reserve_segments free segments (else refused); then ht_ops table
  * operations, 70 % insert / 30 % remove, keys uniform in [1, 2^28].
+ * gate 0 counts free segments as the segment tree's set bits; gate 1 counts
+ * the free memory in segments: those free segments plus the free blocks of
+ * formatted segments (a block of class t is 2^t / blocks_per_segment of a
+ * segment).
  * Counters (device, 16 x u64): [0] admitted, [1] refused, [2] ended, [3]
  * words appended, [4] words refused (a chunk could not be allocated, or
  * the vector is at its 4,210,688-word limit), [5] chunks allocated, [6]
@@ -379,8 +383,11 @@ typedef struct gal_serving gal_serving;
 int gal_serving_create(uint32_t max_sessions, uint64_t seed,
                        const uint32_t *tokens_table_host, gal_serving **out);
 int gal_serving_step(gal_serving *s, uint32_t step, uint32_t arrivals,
-                     uint64_t reserve_segments, uint32_t end_q32, uint32_t ht_ops,
-                     uint64_t *counters_dev, void *stream);
+                     uint64_t reserve_segments, uint32_t gate, uint32_t end_q32,
+                     uint32_t ht_ops, uint64_t *counters_dev, void *stream);
+/* The gate's inputs at the last step: free segments, and free memory in
+ * segments (gate 1's count).  Synchronous. */
+int gal_serving_gate(gal_serving *s, uint64_t *free_segments, uint64_t *free_memory_segments);
 /* Live sessions, words checked, words not equal to (id << 32) | position,
  * and the bytes of the chunks (and chunk indexes) the sessions hold.
  * Synchronous. */

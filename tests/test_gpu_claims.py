@@ -40,10 +40,12 @@ def _lib():
 
 def _instances(gpu, pre16=0):
     """A deterministic GPU instance and the oracle, both with the first
-    `pre16` 16-byte slices of class 0 taken (waves of up to 64)."""
+    `pre16` 16-byte slices of class 0 taken (waves of up to 64); returns
+    them and the offsets of those slices."""
     a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(
         POOL, 42, print_info=False, flags=gpu.GAL_FLAG_DETERMINISTIC, limits=QUICK)
     o = oracle.Allocator(POOL, limits=QUICK)
+    pre = []
     if pre16:
         nsteps = (pre16 + 63) // 64
         steps = np.zeros(nsteps, H.STEP_DTYPE)
@@ -53,7 +55,9 @@ def _instances(gpu, pre16=0):
             steps[k]["arg"][:lanes] = 16
         got = a.run_wave_stream(steps)
         assert np.array_equal(got, o.run_stream(steps))
-    return a, o
+        pre = [int(got[64 * k + j]) for k in range(nsteps) for j in range(min(64, pre16 - 64 * k))]
+        assert sorted(pre) == [16 * i for i in range(pre16)]
+    return a, o, pre
 
 
 def _oracle_claims(o, sizes):
@@ -115,7 +119,7 @@ def test_mixed_classes_match_oracle_claims(gpu, kind, threads):
     n = threads
     pool = [16, 100, 128, 1000, 4000, 300, 2048]
     sizes = [int(x) for x in rng.choice(pool, n)]
-    a, o = _instances(gpu)
+    a, o, _ = _instances(gpu)
     offs = _run(gpu, a, sizes, kind, threads)
     assert np.all(offs != np.uint64(~np.uint64(0)))
     assert _by_class(sizes, offs) == _oracle_claims(o, sizes)
@@ -127,11 +131,11 @@ def test_mixed_classes_match_oracle_claims(gpu, kind, threads):
                                        ("cache_refill_crossing_4095_1GiB", "cache")])
 def test_claim_crossing_slice_4095(gpu, name, kind):
     g = G[name]
-    a, o = _instances(gpu, g["pre_slices_class0"])
+    a, o, pre = _instances(gpu, g["pre_slices_class0"])
     sizes = [g["size"]] * g["requests"]
     offs = _run(gpu, a, sizes, kind, max(64, g["requests"]))
     got = sorted(int(x) for x in offs)
     assert got == _runs(g["runs"]), "hand-derived slices"
     assert got == _oracle_claims(o, sizes)[0], "oracle's coalesced claims"
-    _free_all_and_check_fresh(gpu, a, offs)
+    _free_all_and_check_fresh(gpu, a, list(offs) + pre)
     a.free_on_device()

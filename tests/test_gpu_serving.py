@@ -114,20 +114,25 @@ def test_serving_out_of_memory_fails_fast(gpu):
 def test_serving_full_config(gpu):
     """The serving configuration at full size: 200 GiB pool, Poisson(2000)
     arrivals, geometric(0.01) lifetimes, 2^20 table operations per step on
-    keys in [1, 2^28], 5 % free-segment reserve.  After 60 steps every word
-    reads back, no append was cut short, and teardown drains to fresh."""
+    keys in [1, 2^28], a 10 % free-segment reserve.  Over 700 steps the pool
+    fills, the gate closes (arrivals refused) and opens again; no append is
+    cut short, every word reads back, and teardown drains to fresh."""
     import torch
     from gallatin_amd import serving as SV
     gpu.init_global_allocator(200 << 30, 1234, print_info=False)
     try:
-        r = SV.run(torch, steps=50, warmup=10, seed=1234, max_sessions=1 << 19,
-                   arrivals_mean=2000, p_end=0.01, ht_ops=1 << 20, reserve_fraction=0.05)
+        r = SV.run(torch, steps=50, warmup=650, seed=1234, max_sessions=1 << 19,
+                   arrivals_mean=2000, p_end=0.01, ht_ops=1 << 20, reserve_fraction=0.10,
+                   trace_every=50)
         c = r["counters"]
-        _check_counts(r, 60, 1 << 20)
+        _check_counts(r, 700, 1 << 20)
         assert c["words_refused"] == 0 and c["chunk_failed"] == 0 and c["ht_failed"] == 0
-        assert r["live_sessions"] > 60 * 2000 * 0.7
+        assert c["refused"] > 0, "the admission gate never closed"
+        assert min(t["segments_free"] for t in r["trace"]) >= r["reserve_segments"] // 2
         g = gpu.global_allocator()
         assert g.info()["errors"] == 0
+        g.drain()
+        H.check_fresh(g)
     finally:
         gpu.free_global_allocator()
 
