@@ -88,7 +88,9 @@ class Info(ctypes.Structure):
                 ("segments_free", _u64),
                 ("segments_formatted", _u64), ("segments_multi", _u64),
                 ("tree_owned", _u64 * 24), ("errors", _u64),
-                ("overhead_bytes", _u64), ("pool_base", _u64)]
+                ("overhead_bytes", _u64), ("pool_base", _u64),
+                ("class_segments", _u64 * 24), ("class_blocks_free", _u64 * 24),
+                ("class_blocks_used", _u64 * 24)]
 
 
 STEP_DTYPE = np.dtype([("kind", "<u4"), ("pad", "<u4"), ("mask", "<u8"),
@@ -288,19 +290,22 @@ class Gallatin:
     STAT_NAMES = ("errors", "boot_failed", "segments_set_up", "segments_released",
                   "blocks_returned", "refills", "err_chunk_id", "err_ring_order",
                   "err_block_free", "err_ring_empty", "err_setup", "err_tag",
-                  "err_multi_head", "err_invalid_free", "miss_no_block", "miss_retries")
+                  "err_multi_head", "err_invalid_free", "miss_no_block", "miss_retries",
+                  "ring_waits", "ring_spins")
 
     def stats(self):
         """Event counters (gal_get_stats) by name."""
-        buf = (ctypes.c_uint64 * 16)()
-        _check(lib().gal_get_stats(self.handle, buf, 16))
+        buf = (ctypes.c_uint64 * len(self.STAT_NAMES))()
+        _check(lib().gal_get_stats(self.handle, buf, len(self.STAT_NAMES)))
         return {n: int(buf[i]) for i, n in enumerate(self.STAT_NAMES)}
 
     def info(self):
         i = Info()
         _check(lib().gal_get_info(self.handle, ctypes.byref(i)))
-        d = {f: getattr(i, f) for f, _ in Info._fields_ if f != "tree_owned"}
-        d["tree_owned"] = list(i.tree_owned)[: i.num_trees]
+        arrays = ("tree_owned", "class_segments", "class_blocks_free", "class_blocks_used")
+        d = {f: getattr(i, f) for f, _ in Info._fields_ if f not in arrays}
+        for f in arrays:
+            d[f] = list(getattr(i, f))[: i.num_trees]
         return d
 
     def drain(self):

@@ -538,8 +538,19 @@ int gal_get_info(gal_allocator* a, gal_info* out) {
     return hip_fail(e, "info copy");
   for (uint64_t i = 0; i < h.num_segments; ++i) {
     const uint32_t t = (uint32_t)(st[i] & 0xFFFF);
-    if (t < h.num_trees) out->segments_formatted++;
-    else if (t != kFreeTree) out->segments_multi++;
+    const int32_t active = (int32_t)(st[i] >> 32);
+    if (t < h.num_trees) {
+      out->segments_formatted++;
+      if (active != -1) {  // a closed segment is on its way back (print_guided_fill skips it)
+        const uint64_t nb = h.blocks_per_segment >> t;
+        const uint64_t avail = active < 0 ? 0 : (uint64_t)active + 1;
+        out->class_segments[t]++;
+        out->class_blocks_free[t] += avail;
+        out->class_blocks_used[t] += avail < nb ? nb - avail : 0;
+      }
+    } else if (t != kFreeTree) {
+      out->segments_multi++;
+    }
   }
   unsigned long long stats[kStatCount];
   if ((e = hipMemcpy(stats, h.stats, sizeof(stats), hipMemcpyDeviceToHost)) != hipSuccess)
@@ -570,6 +581,16 @@ int gal_print_info(gal_allocator* a) {
     printf("Tree %u: size %llu, owns %llu/%llu\n", t,
            (unsigned long long)(info.smallest << t), (unsigned long long)info.tree_owned[t],
            (unsigned long long)info.num_segments);
+  // print_usage / print_guided_fill (gallatin.cuh:1946-1990): per class, the
+  // segments formatted for it and their average free and handed-out blocks
+  printf("\nTree usage stats:\n");
+  for (uint32_t t = 0; t < info.num_trees; ++t) {
+    const uint64_t n = info.class_segments[t];
+    const unsigned long long nb = (unsigned long long)(info.blocks_per_segment >> t);
+    printf("Tree %u: %llu live segments | avg available %f / %llu | avg in use %f / %llu\n", t,
+           (unsigned long long)n, n ? 1.0 * info.class_blocks_free[t] / n : 0.0, nb,
+           n ? 1.0 * info.class_blocks_used[t] / n : 0.0, nb);
+  }
   printf("Formatted segments %llu, multi-segment heads %llu, errors %llu\n",
          (unsigned long long)info.segments_formatted, (unsigned long long)info.segments_multi,
          (unsigned long long)info.errors);

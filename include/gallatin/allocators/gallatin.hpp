@@ -137,12 +137,17 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
   atomic_exch(&s->ring[(seg << s->bps_shift) + r], blk);
   release_to_xcd();
   // finalize_free_queue: tickets complete in order
-  for (uint32_t spin = 0; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
+  uint32_t spin = 0;
+  for (; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
     if (spin >= kSpinCap) {
       add_error(s, kStatErrRingOrder);
       break;
     }
     utils::backoff();
+  }
+  if (spin) {
+    add_stat(s, kStatRingWaits, 1);
+    add_stat(s, kStatRingSpins, spin);
   }
   // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) on the segment's fill: a
   // return that brings its free blocks to at least ceil(nb / 10), without

@@ -159,67 +159,92 @@ __host__ inline void print_global_stats_combined() {
 
 // ---- poison (global_allocator.cuh:238-357) ----
 //
-// Layout of a poisoned allocation of n bytes inside a slot of n + 32 + x
-// bytes (x = the slot's padding): [n, x] (16 B), the caller's n bytes, x
-// bytes of pattern (byte i = (char)i), [n, x] (16 B).  The caller gets the
-// address after the first record.
+// A poisoned allocation of n bytes takes the slot a request of n + 32 bytes
+// gets, of S = get_allocated_size(n + 32) bytes, laid out as
+//
+//   [0, 16)                 head record {n, x}
+//   [16, 16 + n)            the caller's bytes (the pointer returned)
+//   [16 + n, 16 + n + x)    x = S - n - 32 bytes of padding, byte i = (char)i
+//   [16 + n + x, S)         tail record {n, x}
+//
+// The records and the pattern are the reference's observable layout; the
+// check reports the first broken part with the reference's messages.
 
-__device__ inline void* global_malloc_poison(uint64_t bytes_needed) {
-  const uint64_t extra_bytes =
-      global_gallatin->get_allocated_size(bytes_needed + 32) - (bytes_needed + 32);
-  void* alloced_memory = global_gallatin->malloc(bytes_needed + 32);
-  if (!alloced_memory) return nullptr;
-  const uint64_t base = reinterpret_cast<uint64_t>(alloced_memory);
-  uint64_t* poison_start = reinterpret_cast<uint64_t*>(base);
-  uint64_t* poison_end = reinterpret_cast<uint64_t*>(base + 16 + bytes_needed + extra_bytes);
-  char* memory_end = reinterpret_cast<char*>(base + 16 + bytes_needed);
-  for (uint64_t i = 0; i < extra_bytes; ++i) memory_end[i] = (char)i;
-  utils::atomic_store(&poison_start[0], bytes_needed);
-  utils::atomic_store(&poison_start[1], extra_bytes);
-  utils::atomic_store(&poison_end[0], bytes_needed);
-  utils::atomic_store(&poison_end[1], extra_bytes);
-  return reinterpret_cast<void*>(base + 16);
+struct poison_record {
+  uint64_t size;     // n, the bytes the caller asked for
+  uint64_t padding;  // x, the pattern bytes between the caller's and the tail
+};
+
+__device__ inline poison_record* poison_head(void* user) {
+  return reinterpret_cast<poison_record*>(static_cast<char*>(user) - sizeof(poison_record));
 }
 
-// True when both size records and the padding pattern are intact; prints
-// which check failed otherwise, as the reference does.
+__device__ inline poison_record* poison_tail(void* user, const poison_record& r) {
+  return reinterpret_cast<poison_record*>(static_cast<char*>(user) + r.size + r.padding);
+}
+
+// Records are written with device-coherent stores, so a checker in another
+// workgroup or kernel sees them.
+__device__ inline void poison_store(poison_record* at, const poison_record& r) {
+  utils::atomic_store(&at->size, r.size);
+  utils::atomic_store(&at->padding, r.padding);
+}
+
+// A failed malloc returns nullptr (the reference writes its records through
+// the null pointer).
+__device__ inline void* global_malloc_poison(uint64_t bytes_needed) {
+  constexpr uint64_t kRecords = 2 * sizeof(poison_record);
+  const uint64_t slot = global_gallatin->get_allocated_size(bytes_needed + kRecords);
+  char* raw = static_cast<char*>(global_gallatin->malloc(bytes_needed + kRecords));
+  if (raw == nullptr) return nullptr;
+  const poison_record rec{bytes_needed, slot - bytes_needed - kRecords};
+  char* user = raw + sizeof(poison_record);
+  char* pad = user + rec.size;
+  for (uint64_t i = 0; i < rec.padding; ++i) pad[i] = (char)i;
+  poison_store(poison_head(user), rec);
+  poison_store(poison_tail(user, rec), rec);
+  return user;
+}
+
+// True when both records and the padding pattern are intact.  The checks
+// run in the reference's order: null pointer (0), head record shape (1),
+// tail size (2), tail padding (3, where the reference's bool function has a
+// bare return), pattern (4).
 __device__ inline bool global_check_poison(void* allocation) {
   if (allocation == nullptr) {
     printf("Poison violated 0\n");
     return false;
   }
-  const uint64_t alloc_as_bytes = reinterpret_cast<uint64_t>(allocation);
-  const uint64_t* poison_start = reinterpret_cast<const uint64_t*>(alloc_as_bytes - 16);
-  const uint64_t bytes_needed = poison_start[0];
-  const uint64_t extra_bytes = poison_start[1];
-  if ((bytes_needed + extra_bytes) % 16 != 0) {
+  const poison_record head = *poison_head(allocation);
+  if ((head.size + head.padding) % 16 != 0) {
     printf("Poison violated 1\n");
     return false;
   }
-  const char* check_region = reinterpret_cast<const char*>(alloc_as_bytes + bytes_needed);
-  const uint64_t* poison_end =
-      reinterpret_cast<const uint64_t*>(alloc_as_bytes + bytes_needed + extra_bytes);
-  if (poison_end[0] != bytes_needed) {
+  const poison_record tail = *poison_tail(allocation, head);
+  if (tail.size != head.size) {
     printf("Poison violated 2\n");
     return false;
   }
-  if (poison_end[1] != extra_bytes) {
+  if (tail.padding != head.padding) {
     printf("Poison violated 3\n");
     return false;
   }
-  for (uint64_t i = 0; i < extra_bytes; ++i) {
-    if (check_region[i] != (char)i) {
-      printf("Poison 4 at %llu bytes after allocation\n", (unsigned long long)extra_bytes);
+  const char* pad = static_cast<const char*>(allocation) + head.size;
+  for (uint64_t i = 0; i < head.padding; ++i) {
+    if (pad[i] != (char)i) {
+      printf("Poison 4 at %llu bytes after allocation\n", (unsigned long long)head.padding);
       return false;
     }
   }
   return true;
 }
 
+// Checks, then frees the whole slot; nullptr is checked (and reported) but
+// not freed (the reference would free address -16).
 __device__ inline void global_free_poison(void* allocation) {
   global_check_poison(allocation);
-  if (!allocation) return;  // the reference would free address -16
-  global_free(reinterpret_cast<void*>(reinterpret_cast<uint64_t>(allocation) - 16));
+  if (allocation == nullptr) return;
+  global_free(poison_head(allocation));
 }
 
 }  // namespace allocators

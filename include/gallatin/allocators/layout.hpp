@@ -124,7 +124,9 @@ enum gal_stat {
   kStatErrInvalidFree = 13, // nullptr, foreign or unformatted-segment frees
   kStatMissNoBlock = 14,    // slice requests failed: no block to be had
   kStatMissRetries = 15,    // slice requests failed: retries used up
-  kStatCount = 16
+  kStatRingWaits = 16,      // block returns that waited for an earlier ticket
+  kStatRingSpins = 17,      // polls of those waits, summed
+  kStatCount = 18
 };
 
 // The instance viewed through the reference's alloc_table accessors

@@ -89,6 +89,12 @@ typedef struct gal_info {
   uint64_t errors;                 /* invalid frees / internal inconsistencies */
   uint64_t overhead_bytes;         /* metadata bytes (calculate_overhead) */
   uint64_t pool_base;              /* device address of the pool */
+  /* print_guided_fill (gallatin.cuh:1958-1990) per size class t: segments
+   * formatted for t and not closed, their blocks not yet handed out
+   * (active count + 1 each), and their blocks handed out (the rest). */
+  uint64_t class_segments[24];
+  uint64_t class_blocks_free[24];
+  uint64_t class_blocks_used[24];
 } gal_info;
 
 /* Fill `cfg` with the reference defaults: Gallatin<16 MiB, 16, 4096>,
@@ -112,7 +118,8 @@ void *gal_device_state(gal_allocator *a);
 gal_allocator *gal_from_device_state(const void *dev_state);
 
 /* print_info (gallatin.cuh:1855-1897): segment-tree fill, per-class owned
- * segments, overhead.  Synchronous. */
+ * segments, the per-class guided fill (print_usage, gallatin.cuh:1946-1990),
+ * overhead.  Synchronous. */
 int gal_print_info(gal_allocator *a);
 
 /* Statistics behind print_info as numbers.  Synchronous. */
@@ -127,8 +134,10 @@ int gal_get_info(gal_allocator *a, gal_info *out);
  * unformatted segment, [9] empty ring entry, [10] segment set-up failed,
  * [11] whole-block tag mismatch, [12] multi-segment head marking failed,
  * [13] invalid frees, [14] slice requests failed for want of a block, [15]
- * slice requests failed after all retries.  Synchronous. */
-#define GAL_STAT_COUNT 16
+ * slice requests failed after all retries, [16] block returns that waited
+ * for an earlier ring ticket to complete, [17] the polls of those waits.
+ * Synchronous. */
+#define GAL_STAT_COUNT 18
 int gal_get_stats(gal_allocator *a, uint64_t *out, uint32_t n);
 
 /* Quiescent drain (no counterpart in the reference): give back every pinned
@@ -455,6 +464,7 @@ int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
                           uint32_t batched, uint32_t workgroups,
                           gal_batch_plan **out);
 int gal_batch_plan_launch(gal_batch_plan *plan, void *stream);
+
 int gal_batch_plan_destroy(gal_batch_plan *plan);
 
 /* ---- measurement kernels (ceilings for the roofline) ---- */

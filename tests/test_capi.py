@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import gallatin_amd
     assert ctypes.sizeof(gallatin_amd.Config) == 8 * 5 + 4 * 10
-    assert ctypes.sizeof(gallatin_amd.Info) == 8 * 5 + 4 * 4 + 8 * 3 + 8 * 24 + 8 * 3
+    assert ctypes.sizeof(gallatin_amd.Info) == 8 * 5 + 4 * 4 + 8 * 3 + 8 * 24 + 8 * 3 + 3 * 8 * 24
     assert gallatin_amd.STEP_DTYPE.itemsize == 8 + 8 + 64 * 8
 
 

@@ -132,6 +132,34 @@ def test_multi_segment_known_answers(gpu):
     a.free_on_device()
 
 
+@pytest.mark.parametrize("seed,pool", [(2, 4 << 30), (3, 256 << 20)])
+def test_fill_statistics_match_oracle(gpu, seed, pool):
+    """print_info's numbers (gal_get_info) in the middle of a deterministic
+    stream, with allocations live: the segment tree's free count and each
+    class tree's fill equal the oracle's tree fills (orc_veb_fill,
+    veb.cuh:1052-1077), and the per-class guided fill (print_guided_fill,
+    gallatin.cuh:1958-1990: segments of the class, blocks available and
+    handed out) equals the same sums over the oracle's segment state."""
+    steps, _ = H.make_stream(seed, 200, mix="all")
+    a = make(gpu, pool)
+    o = oracle.Allocator(pool, limits=QUICK)
+    assert np.array_equal(a.run_wave_stream(steps), o.run_stream(steps))
+    info = a.info()
+    assert info["segments_free"] == o.segment_tree().fill()
+    ids, act = o.segment_state()
+    for t in range(a.num_trees):
+        assert info["tree_owned"][t] == o.sub_tree(t).fill(), t
+        m = (ids == t) & (act != -1)
+        nb = a.blocks_per_segment >> t
+        avail = np.clip(act[m].astype(np.int64) + 1, 0, None)
+        assert info["class_segments"][t] == int(m.sum()), t
+        assert info["class_blocks_free"][t] == int(avail.sum()), t
+        assert info["class_blocks_used"][t] == int(np.clip(nb - avail, 0, None).sum()), t
+    assert sum(info["class_segments"]) > 0 and sum(info["class_blocks_used"]) > 0
+    a.print_info()
+    a.free_on_device()
+
+
 @pytest.mark.parametrize("seed,pool,mix,nsteps", [
     (1, 1 << 30, "slices", 400),
     (2, 4 << 30, "all", 400),
