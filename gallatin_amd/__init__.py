@@ -50,6 +50,7 @@ CAPI_SYMBOLS = [
     "gal_bench_scatter", "gal_bench_tree",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
     "gal_batch_plan_launch", "gal_batch_plan_destroy", "gal_check_batch_stream",
+    "gal_run_stream_static",
     "gal_vector_create", "gal_vector_destroy", "gal_vector_insert", "gal_vector_export",
     "gal_table_create", "gal_table_destroy", "gal_table_insert", "gal_table_query",
     "gal_table_remove", "gal_table_stats", "gal_serving_create", "gal_serving_step",
@@ -143,6 +144,8 @@ def lib():
                                             _u32, ctypes.POINTER(_p)]
         L.gal_batch_plan_launch.argtypes = [_p, _p]
         L.gal_check_batch_stream.argtypes = [_p, _p, _u32, _u64, ctypes.c_int]
+        L.gal_run_stream_static.argtypes = [_p, _p, _p, _u32, _p, _u64, _p, _u32, _u32, _u32,
+                                            _p]
         L.gal_batch_plan_destroy.argtypes = [_p]
         L.gal_vector_create.argtypes = [ctypes.POINTER(_p)]
         L.gal_vector_destroy.argtypes = [_p]
@@ -471,6 +474,18 @@ class BatchPlan:
             self.destroy()
         except Exception:
             pass
+
+
+def run_stream_static(alloc, stream, counters=None, threads=0, batched=True, workgroups=0):
+    """gal_run_stream_static: the persistent form of the stream, run once on
+    the current stream with the library's static workspace (no device
+    allocation)."""
+    ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+    _check(lib().gal_run_stream_static(alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(),
+                                       stream.nbatches, stream.ptrs.data_ptr(), stream.nslots,
+                                       ctr.data_ptr(), threads, _batch_mode(batched), workgroups,
+                                       _stream()))
+    return ctr
 
 
 def run_batches(alloc, stream, counters=None, threads=0, batched=True, workgroups=0):

@@ -15,6 +15,7 @@
 // kernel per operation type (tests/src/gallatin_test.cu:59-112).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -343,6 +344,61 @@ __global__ void k_stream_reset(uint32_t* queue) {
   if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = 0;
 }
 
+// The chunk table of a stream from its batch offsets, the same table
+// gal_batch_plan_create builds on the host: the ops of batch b cut into
+// chunks of at most `per`, in stream order.  One workgroup of 1024 threads
+// takes 1024 batches at a time, one per thread, and places their chunks by
+// an LDS prefix sum of the chunk counts.  Also resets the queue counter.
+__global__ __launch_bounds__(1024) void k_build_chunks(const uint64_t* offsets, uint32_t nbatches,
+                                                      uint32_t per, stream_chunk* chunks,
+                                                      uint32_t* queue) {
+  __shared__ uint32_t part[1024];
+  __shared__ uint32_t base_s;
+  if (threadIdx.x == 0) {
+    base_s = 0;
+    queue[0] = 0;
+  }
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nbatches; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint64_t lo = 0, n = 0;
+    if (b < nbatches) {
+      lo = offsets[b];
+      n = offsets[b + 1] - lo;
+    }
+    const uint32_t c = (uint32_t)((n + per - 1) / per);
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive scan
+      const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const uint32_t at = base_s + part[threadIdx.x] - c;
+    for (uint32_t k = 0; k < c; ++k) {
+      const uint64_t left = n - (uint64_t)k * per;
+      chunks[at + k] = stream_chunk{lo + (uint64_t)k * per, (uint32_t)(left < per ? left : per), b};
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) base_s += part[1023];
+    __syncthreads();
+  }
+}
+
+// Workspace of gal_run_stream_static, the entry point that allocates
+// nothing: the queue counter, a copy of the batch offsets and the chunk
+// table, in one array per device that lives until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (the array's name is a host-side stand-in there).
+constexpr uint32_t kStaticMaxBatches = 65535;
+constexpr uint32_t kStaticMaxChunks = 1u << 17;
+constexpr size_t kScratchOffsets = 256;  // bytes [0, 256): the queue counter
+constexpr size_t kScratchChunks = kScratchOffsets + 8 * (size_t)(kStaticMaxBatches + 1);
+constexpr size_t kScratchBytes = kScratchChunks + sizeof(stream_chunk) * (size_t)kStaticMaxChunks;
+static_assert(sizeof(stream_chunk) == 16, "chunk table entries are 16 bytes");
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / 8];
+
 template <typename K>
 unsigned fill_grid(K kernel, int threads, uint32_t workgroups) {
   if (workgroups) return workgroups;
@@ -397,19 +453,18 @@ struct gal_batch_plan {
 };
 
 // Enqueue reset, the persistent launch and the cleanup on `st`.
-static int enqueue_persistent(gal_batch_plan* plan, const gal_batch_op* ops, uint32_t nchunks,
-                              uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
-                              uint32_t threads, uint32_t batched, uint32_t workgroups,
-                              hipStream_t st) {
-  gal_state* s = plan->a->dev;
-  k_stream_reset<<<1, 64, 0, st>>>(plan->queue);
+static int enqueue_persistent(gal_state* s, const stream_chunk* chunks, uint32_t* queue,
+                              const gal_batch_op* ops, uint32_t nchunks, uint64_t* ptrs,
+                              uint64_t nslots, uint64_t* ctr, uint32_t threads, uint32_t batched,
+                              uint32_t workgroups, hipStream_t st) {
+  k_stream_reset<<<1, 64, 0, st>>>(queue);
   if (nchunks) {
 #define GAL_STREAM_LAUNCH(T, B)                                                              \
   do {                                                                                        \
     unsigned g = fill_grid(k_stream<T, B>, T, workgroups);                                    \
     if (g > nchunks) g = nchunks;                                                             \
-    k_stream<T, B><<<g, T, 0, st>>>(s, ops, plan->chunks, nchunks, plan->queue, ptrs, nslots, \
-                                    ctr, kStreamSpinCap);                                          \
+    k_stream<T, B><<<g, T, 0, st>>>(s, ops, chunks, nchunks, queue, ptrs, nslots, ctr,       \
+                                    kStreamSpinCap);                                          \
   } while (0)
     if (batched == 0)
       GAL_STREAM_LAUNCH(256, 0);
@@ -495,6 +550,28 @@ static int enqueue_stream(gal_allocator* a, const gal_batch_op* ops, const uint6
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "cleanup launch");
   }
+  return GAL_OK;
+}
+
+// The address of `scratch` on the current device, looked up once per
+// device with that device current and kept.  Every call on a device shares
+// the one array.
+static std::mutex g_workspace_mu;
+static char* g_workspace[64];
+
+static int static_workspace(char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(GAL_ERR_INVALID, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_workspace_mu);
+  if (!g_workspace[dev]) {
+    char* ptr = nullptr;
+    e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_workspace[dev] = ptr;
+  }
+  *out = g_workspace[dev];
   return GAL_OK;
 }
 
@@ -620,8 +697,8 @@ int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint6
     gal_batch_plan_destroy(plan);
     return hip_fail(e, "begin capture");
   }
-  rc = enqueue_persistent(plan, ops, (uint32_t)chunks.size(), nbatches, ptrs, nslots, ctr,
-                          threads, batched, workgroups, plan->capture);
+  rc = enqueue_persistent(a->dev, plan->chunks, plan->queue, ops, (uint32_t)chunks.size(), ptrs,
+                          nslots, ctr, threads, batched, workgroups, plan->capture);
   e = hipStreamEndCapture(plan->capture, &plan->graph);
   if (rc == GAL_OK && e != hipSuccess) rc = hip_fail(e, "end capture");
   if (rc == GAL_OK) {
@@ -634,6 +711,37 @@ int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint6
   }
   *out = plan;
   return GAL_OK;
+}
+
+int gal_run_stream_static(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
+                          uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
+                          uint32_t threads, uint32_t batched, uint32_t workgroups, void* stream) {
+  int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads, batched);
+  if (rc != GAL_OK) return rc;
+  if (nbatches > kStaticMaxBatches)
+    return fail(GAL_ERR_INVALID, "the static workspace holds at most 65535 batches; got " +
+                                     std::to_string(nbatches));
+  const uint32_t per = kChunkRounds * (batched ? threads : 256u);
+  uint64_t nchunks = 0;
+  for (uint32_t b = 0; b < nbatches; ++b) nchunks += (offsets[b + 1] - offsets[b] + per - 1) / per;
+  if (nchunks > kStaticMaxChunks)
+    return fail(GAL_ERR_INVALID, "the static workspace holds at most 131072 chunks of " +
+                                     std::to_string(per) + " ops; the stream needs " +
+                                     std::to_string(nchunks));
+  char* sp = nullptr;
+  rc = static_workspace(&sp);
+  if (rc != GAL_OK) return rc;
+  uint32_t* queue = reinterpret_cast<uint32_t*>(sp);
+  uint64_t* offs = reinterpret_cast<uint64_t*>(sp + kScratchOffsets);
+  stream_chunk* chunks = reinterpret_cast<stream_chunk*>(sp + kScratchChunks);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(offs, offsets, 8 * ((size_t)nbatches + 1), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "offsets into the static workspace");
+  if (nbatches) k_build_chunks<<<1, 1024, 0, st>>>(offs, nbatches, per, chunks, queue);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "chunk table launch");
+  return enqueue_persistent(a->dev, chunks, queue, ops, (uint32_t)nchunks, ptrs, nslots, ctr,
+                            threads, batched, workgroups, st);
 }
 
 int gal_batch_plan_launch(gal_batch_plan* plan, void* stream) {

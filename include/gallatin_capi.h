@@ -465,6 +465,22 @@ int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
                           gal_batch_plan **out);
 int gal_batch_plan_launch(gal_batch_plan *plan, void *stream);
 
+/* The persistent form of the stream run once, eagerly on `stream`, without
+ * allocating anything: no counterpart in the reference.  It is for callers
+ * that cannot allocate device memory (HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, requirements
+ * (see gal_batch_plan_create; the slot rule is not checked here) and
+ * results as gal_batch_plan_create + gal_batch_plan_launch.  The chunk
+ * table, a copy of the batch offsets and the queue counter live in one
+ * static device workspace of 2.5 MiB per device, held until the process
+ * ends: at most 65,535 batches and 131,072 chunks of 4 x threads ops
+ * (1,024 ops for batched = 0), else GAL_ERR_INVALID before any device work.
+ * Every call on a device shares that workspace: calls that overlap on one
+ * device race, so a caller orders them (one stream, or a synchronisation). */
+int gal_run_stream_static(gal_allocator *a, const gal_batch_op *ops_dev,
+                          const uint64_t *offsets, uint32_t nbatches, uint64_t *ptrs_dev,
+                          uint64_t nslots, uint64_t *counters_dev, uint32_t threads,
+                          uint32_t batched, uint32_t workgroups, void *stream);
 int gal_batch_plan_destroy(gal_batch_plan *plan);
 
 /* ---- measurement kernels (ceilings for the roofline) ---- */
