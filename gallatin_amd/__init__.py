@@ -294,7 +294,7 @@ class Gallatin:
                   "blocks_returned", "refills", "err_chunk_id", "err_ring_order",
                   "err_block_free", "err_ring_empty", "err_setup", "err_tag",
                   "err_multi_head", "err_invalid_free", "miss_no_block", "miss_retries",
-                  "ring_waits", "ring_spins")
+                  "ring_runs", "ring_run_tickets")
 
     def stats(self):
         """Event counters (gal_get_stats) by name."""

@@ -49,7 +49,7 @@ __global__ void k_init_segments(gal_state* s) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= s->num_segments) return;
   s->seg_state[i] = kSegFreeState;  // gallatin_init_counters_kernel
-  s->seg_queue[i] = gal_seg_queue{0, 0, 0, 0};
+  s->seg_queue[i] = gal_seg_queue{0, 0, 0};
 }
 
 __global__ void k_init_blocks(gal_state* s) {

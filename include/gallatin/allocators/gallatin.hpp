@@ -125,49 +125,31 @@ __device__ inline void clear_chunk_id(gal_state* s, uint64_t seg) {
 
 // ----------------------------------------------------------- per lane ops
 
-// Segment ring return, gallatin.cuh:1637-1726 return_block with
-// memory_table.cuh:1158-1241 reserve_segment_slot / finish_freeing_block.
-// `h` is the segment's home partition, `my_p` the caller's.
-__device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t seg,
-                                          uint32_t t, uint32_t h, uint32_t my_p) {
-  const uint64_t nb = nb_of(s, t);
-  gal_seg_queue* q = &s->seg_queue[seg];
-  const uint32_t pos = atomic_add(&q->q_free, 1u);
-  const uint32_t r = pos & (uint32_t)(nb - 1);
-  atomic_exch(&s->ring[(seg << s->bps_shift) + r], blk);
-  release_to_xcd();
-  // finalize_free_queue: tickets complete in order
-  uint32_t spin = 0;
-  for (; atomic_cas(&q->q_final, pos, pos + 1) != pos; ++spin) {
-    if (spin >= kSpinCap) {
-      add_error(s, kStatErrRingOrder);
-      break;
-    }
-    utils::backoff();
-  }
-  if (spin) {
-    add_stat(s, kStatRingWaits, 1);
-    add_stat(s, kStatRingSpins, spin);
-  }
-  // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) on the segment's fill: a
-  // return that brings its free blocks to at least ceil(nb / 10), without
-  // completing it, puts it back in its class tree when it is not there.  The
-  // reference compares the ring position mod nb instead ("TODO - make this
-  // actually calculate fill", memory_table.cuh:1181), which hides a segment
-  // emptied at ring position k until its ring wraps back to 10 %, about half
-  // of its blocks later.  The insert comes before this return's count
-  // increment, so the segment cannot be released in between.
+// Ring tickets [f, f + k) of segment seg have just been completed: their k
+// blocks become available to get_block, and the segment goes back to its
+// class tree or, when every block is back, to the segment tree
+// (gallatin.cuh:1646-1722, memory_table.cuh:1226-1241).
+__device__ inline void complete_run(gal_state* s, uint64_t seg, uint32_t t, uint32_t h,
+                                    uint32_t my_p, uint64_t nb, uint32_t k) {
+  // REREGISTER_CUTOFF 0.1 (gallatin.cuh:1646) on the segment's fill: returns
+  // that bring its free blocks to at least ceil(nb / 10), without completing
+  // it, put it back in its class tree when it is not there.  The reference
+  // compares the ring position mod nb instead ("TODO - make this actually
+  // calculate fill", memory_table.cuh:1181), which hides a segment emptied
+  // at ring position k until its ring wraps back to 10 %, about half of its
+  // blocks later.  The insert comes before the count increment, so the
+  // segment cannot be released in between.
   {
     const gal_tree& ct = class_tree(s, t, h);
     if (!veb::query(ct, seg)) {
       const int64_t f_after =
-          (int64_t)(int32_t)(atomic_load(&s->seg_state[seg]) >> 32) + 2;
+          (int64_t)(int32_t)(atomic_load(&s->seg_state[seg]) >> 32) + 1 + k;
       if (10 * f_after >= (int64_t)nb && f_after < (int64_t)nb) veb::insert(ct, seg);
     }
   }
-  const uint64_t old = atomic_add(&s->seg_state[seg], kActiveOne);
+  const uint64_t old = atomic_add(&s->seg_state[seg], (uint64_t)k * kActiveOne);
   const int32_t rid = (int32_t)(old >> 32);
-  if ((int64_t)rid == (int64_t)nb - 2) {
+  if ((int64_t)rid + k == (int64_t)nb - 1) {
     if (my_p != h) {
       // every block is back but this wave is not on the home XCD, whose L2
       // may still hold lines of the segment: keep it formatted for its home
@@ -185,6 +167,80 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
       veb::insert(s->seg_tree, seg);
       add_stat(s, kStatSegmentsReleased, 1);
     }
+  }
+}
+
+// Segment ring return, gallatin.cuh:1637-1726 return_block with
+// memory_table.cuh:1158-1241 reserve_segment_slot / finish_freeing_block.
+// `h` is the segment's home partition, `my_p` the caller's.
+//
+// Tickets still complete in order (the free count never covers a ring
+// entry that is not written yet), but returners do not wait for each other.
+// The reference's returner spins until q_final reaches its own ticket and
+// then advances it by one (finalize_free_queue, memory_table.cuh:906-910),
+// so returns into one segment form a chain of dependent compare-and-swaps.
+// Here the completion state is one word, q_done = (frontier << 32) | a
+// written-mask over tickets frontier .. frontier + 31, ticket x at bit
+// x mod 32.  A returner writes its entry, then sets its ticket's bit with
+// one atomic or; whoever then sees the frontier's bit set moves the frontier
+// over the whole run of set bits, clearing them, with one compare-and-swap
+// and makes the run available.  A returner whose ticket is not at the
+// frontier leaves: the writer of the frontier ticket sets its bit later and
+// then sees every bit set before it, because all of this happens on the one
+// word.  Only a ticket 32 or more ahead of the frontier waits, for the
+// frontier to move.  Marking with an or (not a compare-and-swap) lets the
+// lanes of a wave that return into one segment mark in one round.
+__device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t seg,
+                                          uint32_t t, uint32_t h, uint32_t my_p) {
+  const uint64_t nb = nb_of(s, t);
+  gal_seg_queue* q = &s->seg_queue[seg];
+  const uint32_t pos = atomic_add(&q->q_free, 1u);
+  atomic_exch(&s->ring[(seg << s->bps_shift) + (pos & (uint32_t)(nb - 1))], blk);
+  release_to_xcd();  // the entry is written before its ticket is marked
+  // One loop marks this ticket and then completes runs at the frontier, so
+  // that lanes of one wave returning into the same segment help each other:
+  // a lane waiting for the frontier to come within reach iterates beside
+  // the lane that moves it (two loops in a row would hold the mover at the
+  // end of the first until the waiter gave up).
+  uint64_t w = atomic_load(&q->q_done);
+  bool marked = false;
+  for (uint32_t spin = 0;; ++spin) {
+    if (!marked) {
+      if (pos - (uint32_t)(w >> 32) >= 32) {
+        if (spin >= kSpinCap) {  // the frontier never came within reach
+          add_error(s, kStatErrRingOrder);
+          return;
+        }
+        utils::backoff();
+        w = atomic_load(&q->q_done);
+        continue;
+      }
+      // Within reach, the ticket's bit (pos mod 32) is free: the frontier
+      // only moves forward, and moving it over ticket pos - 32 cleared it.
+      const uint64_t bit = (uint64_t)1 << (pos & 31);
+      w = atomic_or(&q->q_done, bit) | bit;
+      marked = true;
+    }
+    // complete the run of written tickets at the frontier, if there is one:
+    // the window's bits rotated so that the frontier's comes first
+    const uint32_t f = (uint32_t)(w >> 32), r = f & 31, bits = (uint32_t)w;
+    const uint32_t rot = r ? (bits >> r) | (bits << (32 - r)) : bits;
+    if (!(rot & 1u)) return;
+    const uint32_t k = rot == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~rot);
+    const uint32_t run = k == 32 ? 0xFFFFFFFFu : (1u << k) - 1u;
+    const uint32_t cleared = r ? (run << r) | (run >> (32 - r)) : run;
+    const uint64_t next = ((uint64_t)(f + k) << 32) | (uint64_t)(bits & ~cleared);
+    const uint64_t seen = atomic_cas(&q->q_done, w, next);
+    if (seen != w) {
+      w = seen;  // another returner moved the frontier or set a bit; look again
+      continue;
+    }
+    if (k > 1) {
+      add_stat(s, kStatRingRuns, 1);
+      add_stat(s, kStatRingRunTickets, k);
+    }
+    complete_run(s, seg, t, h, my_p, nb, k);
+    w = next;
   }
 }
 
@@ -338,7 +394,7 @@ __device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t, uin
     gal_seg_queue* q = &s->seg_queue[seg];
     atomic_store(&q->q_malloc, 0u);
     atomic_store(&q->q_free, 0u);
-    atomic_store(&q->q_final, 0u);
+    atomic_store(&q->q_done, (uint64_t)0);
   }
   release_to_xcd();
   if (wave::is_first()) {

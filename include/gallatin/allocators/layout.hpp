@@ -100,8 +100,10 @@ struct gal_tree {
 struct gal_seg_queue {
   uint32_t q_malloc;  // ring read position   (memory_table.cuh:244)
   uint32_t q_free;    // ring write ticket    (memory_table.cuh:247)
-  uint32_t q_final;   // ring write completion (memory_table.cuh:249)
-  uint32_t pad;
+  // ring write completion (memory_table.cuh:249): the first incomplete
+  // ticket in the high half, the written-mask of it and the next 31 tickets
+  // in the low half
+  uint64_t q_done;
 };
 
 // stats[] indices.  kStatErrors is the sum of the kinds from kStatErrFirst
@@ -124,8 +126,8 @@ enum gal_stat {
   kStatErrInvalidFree = 13, // nullptr, foreign or unformatted-segment frees
   kStatMissNoBlock = 14,    // slice requests failed: no block to be had
   kStatMissRetries = 15,    // slice requests failed: retries used up
-  kStatRingWaits = 16,      // block returns that waited for an earlier ticket
-  kStatRingSpins = 17,      // polls of those waits, summed
+  kStatRingRuns = 16,       // ring completions that covered several tickets
+  kStatRingRunTickets = 17, // tickets those completions covered
   kStatCount = 18
 };
 

@@ -134,9 +134,9 @@ int gal_get_info(gal_allocator *a, gal_info *out);
  * unformatted segment, [9] empty ring entry, [10] segment set-up failed,
  * [11] whole-block tag mismatch, [12] multi-segment head marking failed,
  * [13] invalid frees, [14] slice requests failed for want of a block, [15]
- * slice requests failed after all retries, [16] block returns that waited
- * for an earlier ring ticket to complete, [17] the polls of those waits.
- * Synchronous. */
+ * slice requests failed after all retries, [16] segment-ring completions
+ * that covered more than one returned block, [17] the returned blocks they
+ * covered.  Synchronous. */
 #define GAL_STAT_COUNT 18
 int gal_get_stats(gal_allocator *a, uint64_t *out, uint32_t n);
 
