@@ -38,7 +38,9 @@ def biggest_for(segment_bytes):
 
 
 def measure(alloc, stream, batched, threads, reps=3, warmup=1):
-    """Time `reps` graph replays of `stream`; returns a result dict."""
+    """Time `reps` graph replays of `stream`; returns a result dict.  Its
+    `events` are the allocator's event counters (gal_get_stats) that moved
+    during the timed replays, summed over them."""
     import torch
     ctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
     plan = BatchPlan(alloc, stream, ctr, threads=threads, batched=batched)
@@ -48,12 +50,14 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
         torch.cuda.synchronize()
         ctr.zero_()
         torch.cuda.synchronize()
+        s0 = alloc.stats()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
         evs[0].record()
         for k in range(reps):
             plan.launch()
             evs[k + 1].record()
         torch.cuda.synchronize()
+        s1 = alloc.stats()
     finally:
         plan.destroy()
     c = ctr.cpu().tolist()
@@ -71,6 +75,7 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
         "misses": c[CTR_MISSES],          # over all reps
         "violations": c[CTR_BAD] + c[CTR_TIMEOUTS] + (0 if ok else 1),
         "pairs_per_step": c[CTR_PAIRS] / reps,
+        "events": {k: s1[k] - s0[k] for k in s1 if s1[k] != s0[k]},
     }
 
 

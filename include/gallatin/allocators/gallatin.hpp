@@ -179,17 +179,27 @@ __device__ inline void complete_run(gal_state* s, uint64_t seg, uint32_t t, uint
 // The reference's returner spins until q_final reaches its own ticket and
 // then advances it by one (finalize_free_queue, memory_table.cuh:906-910),
 // so returns into one segment form a chain of dependent compare-and-swaps.
-// Here the completion state is one word, q_done = (frontier << 32) | a
-// written-mask over tickets frontier .. frontier + 31, ticket x at bit
-// x mod 32.  A returner writes its entry, then sets its ticket's bit with
-// one atomic or; whoever then sees the frontier's bit set moves the frontier
-// over the whole run of set bits, clearing them, with one compare-and-swap
-// and makes the run available.  A returner whose ticket is not at the
-// frontier leaves: the writer of the frontier ticket sets its bit later and
-// then sees every bit set before it, because all of this happens on the one
-// word.  Only a ticket 32 or more ahead of the frontier waits, for the
-// frontier to move.  Marking with an or (not a compare-and-swap) lets the
-// lanes of a wave that return into one segment mark in one round.
+// Here the completion state is one word, q_done:
+//   bits 63..56  generation, bumped by every set-up of the segment
+//   bits 55..32  frontier: the first incomplete ticket, mod 2^24
+//   bits 31..0   written-mask over tickets frontier .. frontier + 31,
+//                ticket x at bit x mod 32
+// A returner writes its entry, then sets its ticket's bit with one atomic
+// or.  If the frontier's bit is then set, it moves the frontier over the
+// whole run of set bits, clearing them, with one compare-and-swap, and makes
+// the run available.  Otherwise it leaves: the frontier only ever stops on
+// a clear bit, so the frontier's bit is set by that ticket's own returner,
+// after the frontier got there, and that returner sees it set and moves the
+// frontier over every set bit after it.  A returner also leaves once the
+// frontier has passed its own ticket, so nothing touches the word of a
+// segment after its last block came back (the generation makes a
+// compare-and-swap from before a new set-up fail).  Only a ticket 32 or
+// more ahead of the frontier waits, for the frontier to move.  Marking with
+// an or, not a compare-and-swap, lets the lanes of a wave that return into
+// one segment mark in one round.
+constexpr uint32_t kDoneTickets = 0xFFFFFFu;
+constexpr int kDoneGenShift = 56;
+
 __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t seg,
                                           uint32_t t, uint32_t h, uint32_t my_p) {
   const uint64_t nb = nb_of(s, t);
@@ -197,16 +207,20 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
   const uint32_t pos = atomic_add(&q->q_free, 1u);
   atomic_exch(&s->ring[(seg << s->bps_shift) + (pos & (uint32_t)(nb - 1))], blk);
   release_to_xcd();  // the entry is written before its ticket is marked
-  // One loop marks this ticket and then completes runs at the frontier, so
-  // that lanes of one wave returning into the same segment help each other:
-  // a lane waiting for the frontier to come within reach iterates beside
-  // the lane that moves it (two loops in a row would hold the mover at the
-  // end of the first until the waiter gave up).
+  // One loop waits for reach, marks this ticket and completes runs, so that
+  // lanes of one wave returning into the same segment help each other: a
+  // lane waiting for the frontier to come within reach iterates beside the
+  // lane that moves it (two loops in a row would hold the mover at the end
+  // of the first until the waiter gave up).
   uint64_t w = atomic_load(&q->q_done);
   bool marked = false;
   for (uint32_t spin = 0;; ++spin) {
+    const uint32_t f = (uint32_t)(w >> 32) & kDoneTickets;
+    // distance of this ticket from the frontier; from 32 on after marking
+    // it means the frontier has passed it
+    const uint32_t ahead = (pos - f) & kDoneTickets;
     if (!marked) {
-      if (pos - (uint32_t)(w >> 32) >= 32) {
+      if (ahead >= 32) {
         if (spin >= kSpinCap) {  // the frontier never came within reach
           add_error(s, kStatErrRingOrder);
           return;
@@ -220,16 +234,19 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
       const uint64_t bit = (uint64_t)1 << (pos & 31);
       w = atomic_or(&q->q_done, bit) | bit;
       marked = true;
+      continue;
     }
-    // complete the run of written tickets at the frontier, if there is one:
+    if (ahead >= 32) return;  // another returner completed this ticket
     // the window's bits rotated so that the frontier's comes first
-    const uint32_t f = (uint32_t)(w >> 32), r = f & 31, bits = (uint32_t)w;
+    const uint32_t r = f & 31, bits = (uint32_t)w;
     const uint32_t rot = r ? (bits >> r) | (bits << (32 - r)) : bits;
-    if (!(rot & 1u)) return;
+    if (!(rot & 1u)) return;  // the frontier ticket's returner takes over
     const uint32_t k = rot == 0xFFFFFFFFu ? 32u : (uint32_t)__builtin_ctz(~rot);
     const uint32_t run = k == 32 ? 0xFFFFFFFFu : (1u << k) - 1u;
     const uint32_t cleared = r ? (run << r) | (run >> (32 - r)) : run;
-    const uint64_t next = ((uint64_t)(f + k) << 32) | (uint64_t)(bits & ~cleared);
+    const uint64_t next = (w & ~(((uint64_t)1 << kDoneGenShift) - 1)) |
+                          ((uint64_t)((f + k) & kDoneTickets) << 32) |
+                          (uint64_t)(bits & ~cleared);
     const uint64_t seen = atomic_cas(&q->q_done, w, next);
     if (seen != w) {
       w = seen;  // another returner moved the frontier or set a bit; look again
@@ -239,8 +256,10 @@ __device__ __noinline__ void return_block(gal_state* s, uint32_t blk, uint64_t s
       add_stat(s, kStatRingRuns, 1);
       add_stat(s, kStatRingRunTickets, k);
     }
+    // the run stopped at a clear bit (or cleared the whole window), so the
+    // new frontier's bit is clear: its own returner will find it
     complete_run(s, seg, t, h, my_p, nb, k);
-    w = next;
+    return;
   }
 }
 
@@ -394,7 +413,9 @@ __device__ inline bool setup_segment(gal_state* s, uint64_t seg, uint32_t t, uin
     gal_seg_queue* q = &s->seg_queue[seg];
     atomic_store(&q->q_malloc, 0u);
     atomic_store(&q->q_free, 0u);
-    atomic_store(&q->q_done, (uint64_t)0);
+    // a new generation, frontier 0, nothing marked
+    const uint64_t gen = (atomic_load(&q->q_done) >> kDoneGenShift) + 1;
+    atomic_store(&q->q_done, gen << kDoneGenShift);
   }
   release_to_xcd();
   if (wave::is_first()) {

@@ -100,9 +100,10 @@ struct gal_tree {
 struct gal_seg_queue {
   uint32_t q_malloc;  // ring read position   (memory_table.cuh:244)
   uint32_t q_free;    // ring write ticket    (memory_table.cuh:247)
-  // ring write completion (memory_table.cuh:249): the first incomplete
-  // ticket in the high half, the written-mask of it and the next 31 tickets
-  // in the low half
+  // ring write completion (memory_table.cuh:249): set-up generation in bits
+  // 63..56, the first incomplete ticket mod 2^24 in bits 55..32, and in the
+  // low half the written-mask of it and the next 31 tickets (ticket x at
+  // bit x mod 32)
   uint64_t q_done;
 };
 
