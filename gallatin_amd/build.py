@@ -6,7 +6,9 @@ Sources: gallatin_amd/csrc/*.hip, headers under include/.  The libraries are
 written next to this file so they travel with the source tree.
 """
 import glob
+import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -15,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgallatin_hip.so")
+# per-kernel registers, scratch and occupancy reported by the compiler
+RESOURCES = os.path.join(HERE, "_build", "kernel_resources.json")
 CPU_LIB = os.path.join(HERE, "libgallatin_cpu.so")
 ARCH = "gfx950"
 
@@ -46,21 +50,31 @@ def build_hip(verbose=False, jobs=4):
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
+    resources = {}
+
+    def finish(src, proc):
+        # the remarks go to the resource table, other diagnostics through
+        _, err = proc.communicate()
+        resources.update(parse_resource_usage(err))
+        sys.stderr.write("".join(l + "\n" for l in err.splitlines()
+                                 if "kernel-resource-usage" not in l))
+        if proc.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}")
+
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [_hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-I", os.path.join(HERE, "csrc"),
-               "-c", src, "-o", obj]
+        cmd = [_hipcc(), *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage",
+               "-I", INCLUDE, "-I", os.path.join(HERE, "csrc"), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd)))
+        procs.append((src, subprocess.Popen(cmd, stderr=subprocess.PIPE, text=True)))
         if len(procs) >= jobs:
-            s, p = procs.pop(0)
-            if p.wait() != 0:
-                raise RuntimeError(f"hipcc failed on {s}")
-    for s, p in procs:
-        if p.wait() != 0:
-            raise RuntimeError(f"hipcc failed on {s}")
+            finish(*procs.pop(0))
+    for src, proc in procs:
+        finish(src, proc)
+    with open(RESOURCES, "w") as f:
+        json.dump(resources, f, indent=1, sort_keys=True)
     tmp = LIB + ".tmp"
     # export only the C interface (exports.map): the host-side handles of
     # template kernels instantiated from the public headers stay local, so
@@ -70,6 +84,24 @@ def build_hip(verbose=False, jobs=4):
          verbose)
     os.replace(tmp, LIB)
     return LIB
+
+
+def parse_resource_usage(text):
+    """Kernel name -> {"vgprs", "agprs", "sgprs", "scratch", "occupancy"}
+    from hipcc's -Rpass-analysis=kernel-resource-usage remarks."""
+    keys = {"VGPRs": "vgprs", "AGPRs": "agprs", "TotalSGPRs": "sgprs",
+            "ScratchSize [bytes/lane]": "scratch", "Occupancy [waves/SIMD]": "occupancy"}
+    out, name = {}, None
+    for line in text.splitlines():
+        m = re.search(r"remark:\s+Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            out[name] = {}
+            continue
+        m = re.search(r"remark:\s+([A-Za-z /\[\]]+): (\d+) \[", line)
+        if m and name and m.group(1).strip() in keys:
+            out[name][keys[m.group(1).strip()]] = int(m.group(2))
+    return out
 
 
 def build_cpu(verbose=False):
