@@ -321,6 +321,9 @@ def zipf_sweep(args, G, torch):
         "p99_step_ms": best["p99_step_ms"],
         "best": {k: best[k] for k in keys},
         "misses": best["misses"], "violations": best["violations"],
+        # the stream kernel's wave time: malloc rounds, frees waiting for the
+        # entry an earlier chunk's malloc publishes, frees, the rest
+        "wave_time": best.get("wave_time"),
         "plain_same_geometry": plain[0]["pairs_per_s"] if plain else None,
         "plain_best": plain_best["pairs_per_s"],
         "batched_over_plain": best["pairs_per_s"] / plain_best["pairs_per_s"],
@@ -589,7 +592,8 @@ def compact(out):
     if bz:
         w["batched_zipf"] = dict(
             _pick(bz, "value", "ms_per_step", "p50_step_ms", "p99_step_ms", "best", "misses",
-                  "violations", "plain_best", "batched_over_plain", "outside_grid", "error"),
+                  "violations", "wave_time", "plain_best", "batched_over_plain",
+                  "outside_grid", "error"),
             atomic_frac=bz.get("roofline", {}).get("atomic_frac"),
             cpu_glibc=(out.get("cpu_glibc_batched_zipf") or {}).get("value"))
     sv = out.get("serving")

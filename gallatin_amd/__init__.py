@@ -31,7 +31,8 @@ GAL_FLAG_LAZY_BOOT = 16
 
 CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
 CTR_TIMEOUTS, CTR_FREES_SKIPPED = 5, 6
-CTR_COUNT = 8
+CTR_WAIT_TICKS, CTR_WAVE_TICKS, CTR_WAIT_POLLS, CTR_MALLOC_TICKS, CTR_FREE_TICKS = 7, 8, 9, 10, 11
+CTR_COUNT = 12
 
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32

@@ -22,8 +22,9 @@ import itertools
 
 import numpy as np
 
-from . import (CTR_BAD, CTR_COUNT, CTR_FREES_SKIPPED, CTR_MALLOCS, CTR_MISSES, CTR_PAIRS,
-               CTR_TIMEOUTS, BatchPlan)
+from . import (CTR_BAD, CTR_COUNT, CTR_FREE_TICKS, CTR_FREES_SKIPPED, CTR_MALLOC_TICKS,
+               CTR_MALLOCS, CTR_MISSES, CTR_PAIRS, CTR_TIMEOUTS, CTR_WAIT_POLLS, CTR_WAIT_TICKS,
+               CTR_WAVE_TICKS, BatchPlan)
 from . import Gallatin, GallatinError
 
 SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
@@ -40,7 +41,10 @@ def biggest_for(segment_bytes):
 def measure(alloc, stream, batched, threads, reps=3, warmup=1):
     """Time `reps` graph replays of `stream`; returns a result dict.  Its
     `events` are the allocator's event counters (gal_get_stats) that moved
-    during the timed replays, summed over them."""
+    during the timed replays, summed over them; `wave_time` splits the
+    stream kernel's wave time into malloc rounds, waits of frees for their
+    entries, the frees themselves and the rest (queue, decode), as shares;
+    `wait_polls` counts those waits' polls per replay."""
     import torch
     ctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
     plan = BatchPlan(alloc, stream, ctr, threads=threads, batched=batched)
@@ -76,7 +80,19 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
         "violations": c[CTR_BAD] + c[CTR_TIMEOUTS] + (0 if ok else 1),
         "pairs_per_step": c[CTR_PAIRS] / reps,
         "events": {k: s1[k] - s0[k] for k in s1 if s1[k] != s0[k]},
+        "wave_time": _wave_time(c),
+        "wait_polls": c[CTR_WAIT_POLLS] / reps,
     }
+
+
+def _wave_time(c):
+    life = c[CTR_WAVE_TICKS]
+    if not life:
+        return None
+    parts = {"malloc": c[CTR_MALLOC_TICKS], "wait": c[CTR_WAIT_TICKS], "free": c[CTR_FREE_TICKS]}
+    out = {k: v / life for k, v in parts.items()}
+    out["other"] = 1.0 - sum(out.values())
+    return out
 
 
 def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,

@@ -39,18 +39,36 @@ __device__ inline uint64_t wave_total(uint64_t v) {
 
 struct op_counts {
   uint64_t pairs = 0, misses = 0, bad = 0, bytes = 0, mallocs = 0, skipped = 0;
+  uint64_t polls = 0;  // polls of frees that found their entry unpublished
+  // wall-clock ticks (wall_clock64, a scalar clock: the same on every lane of
+  // a wave) the wave spent in malloc rounds, waiting for entries and
+  // freeing, and its clock at kernel entry; only k_stream keeps them
+  uint64_t malloc_ticks = 0, wait_ticks = 0, free_ticks = 0, start = wall_clock64();
 };
 
-// One atomic per counter per wave (every lane of the wave active).
-__device__ inline void flush_counts(uint64_t* ctr, const op_counts& c) {
+// One atomic per counter per wave (every lane of the wave active).  The
+// wave's time split is added once per wave (lane 0's clock readings), with
+// its lifetime when `timed`.
+__device__ inline void flush_counts(uint64_t* ctr, const op_counts& c, bool timed = false) {
   const uint64_t v[5] = {wave_total(c.pairs), wave_total(c.misses), wave_total(c.bad),
                          wave_total(c.bytes), wave_total(c.mallocs)};
   const uint64_t sk = wave_total(c.skipped);
+  const uint64_t polls = wave_total(c.polls);
+  const uint64_t life = wall_clock64() - c.start;
   if (wave::lane() == 0) {
 #pragma unroll
     for (int i = 0; i < 5; ++i)
       if (v[i]) utils::atomic_add(&ctr[i], v[i]);
     if (sk) utils::atomic_add(&ctr[GAL_CTR_FREES_SKIPPED], sk);
+    if (polls) utils::atomic_add(&ctr[GAL_CTR_WAIT_POLLS], polls);
+    if (timed) {
+      const uint64_t t[4] = {c.wait_ticks, life, c.malloc_ticks, c.free_ticks};
+      const int at[4] = {GAL_CTR_WAIT_TICKS, GAL_CTR_WAVE_TICKS, GAL_CTR_MALLOC_TICKS,
+                         GAL_CTR_FREE_TICKS};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (t[i]) utils::atomic_add(&ctr[at[i]], t[i]);
+    }
   }
 }
 
@@ -101,6 +119,7 @@ __device__ inline void* wait_free(uint64_t* ptrs, uint32_t slot, op_counts& c,
     }
     __builtin_amdgcn_s_sleep(4);
     v = utils::atomic_load(&ptrs[slot]);
+    c.polls += 1;
   }
   utils::atomic_store(&ptrs[slot], (uint64_t)0);
   if (v == kMallocFailed) {
@@ -306,6 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     // them round by round instead (malloc, then wait and free, per round)
     // held a chunk's later mallocs behind its own earlier waits, so waits
     // chained from chunk to chunk through the stream.
+    const uint64_t t0 = wall_clock64();
     for (uint32_t r = 0; r < rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
       bool want, is_free;
@@ -321,6 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
         np = detail::malloc_impl(s, size);
       give_op(np, want, slot, ptrs, c);
     }
+    c.malloc_ticks += wall_clock64() - t0;
     // Frees go through the wave path in every form: a workgroup-wide free
     // would hold every wave behind the slowest wait.
     for (uint32_t r = 0; r < rounds; ++r) {
@@ -332,12 +353,16 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
         is_free = op.kind == 1 && op.slot < nslots;  // invalid ops were counted above
         slot = op.slot;
       }
+      const uint64_t t1 = wall_clock64();
       void* old = is_free ? wait_free(ptrs, slot, c, spin_cap, ctr) : nullptr;
+      const uint64_t t2 = wall_clock64();
       if (old) detail::free_impl(s, old);
+      c.wait_ticks += t2 - t1;
+      c.free_ticks += wall_clock64() - t2;
     }
   }
   if (kMode == 2) cache::cache_flush(s, sc);
-  flush_counts(ctr, c);
+  flush_counts(ctr, c, true);
 }
 
 __global__ void k_stream_reset(uint32_t* queue) {

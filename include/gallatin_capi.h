@@ -201,14 +201,21 @@ typedef struct gal_wave_step {
 int gal_run_wave_stream(gal_allocator *a, const gal_wave_step *steps_dev,
                         uint64_t nsteps, uint64_t *results_dev, void *stream);
 
-/* Counters written by the workload kernels (device array, 8 x u64):
- * [0] pairs (successful malloc+free), [1] misses (nullptr), [2] pattern
- * mismatches, [3] payload bytes written+read, [4] mallocs issued, [5]
- * bounded waits that ran out (batched streams), [6] frees of a table entry
- * that held nullptr (batched streams: the malloc had failed). */
+/* Counters written by the workload kernels (device array, GAL_CTR_COUNT x
+ * u64): [0] pairs (successful malloc+free), [1] misses (nullptr), [2]
+ * pattern mismatches, [3] payload bytes written+read, [4] mallocs issued,
+ * [5] bounded waits that ran out (batched streams), [6] frees of a table
+ * entry that held nullptr (batched streams: the malloc had failed).
+ * Batched streams also count [9] polls of frees whose entry was not yet
+ * published.  The persistent stream kernel adds, per wave, wall-clock ticks
+ * (wall_clock64) spent [7] waiting for entries, [10] in its malloc rounds
+ * and [11] freeing, and [8] its lifetime, so [7] / [8] is the share of the
+ * kernel's wave time spent waiting. */
 enum { GAL_CTR_PAIRS = 0, GAL_CTR_MISSES = 1, GAL_CTR_BAD = 2,
        GAL_CTR_BYTES = 3, GAL_CTR_MALLOCS = 4, GAL_CTR_TIMEOUTS = 5,
-       GAL_CTR_FREES_SKIPPED = 6, GAL_CTR_COUNT = 8 };
+       GAL_CTR_FREES_SKIPPED = 6, GAL_CTR_WAIT_TICKS = 7, GAL_CTR_WAVE_TICKS = 8,
+       GAL_CTR_WAIT_POLLS = 9, GAL_CTR_MALLOC_TICKS = 10, GAL_CTR_FREE_TICKS = 11,
+       GAL_CTR_COUNT = 12 };
 
 /* Plain method: nthreads threads each malloc(size), write their uint32 id,
  * read it back and free (config "plain small"; gallatin_test.cu:59-112 in

@@ -1,7 +1,8 @@
 """The entry point that allocates nothing (gal_run_stream_static) against
 the persistent plan it varies (gal_batch_plan_create + launch): on the same
-Zipf stream its counters equal the plan's exactly (every malloc issued,
-every successful one freed once with its tag intact, no wait ran out), the
+Zipf stream its event counters equal the plan's exactly (every malloc
+issued, every successful one freed once with its tag intact, no wait ran
+out; the wall-clock counters and wait polls differ from run to run), the
 pointer table ends empty and the drained instance is fresh.  Two calls in a
 row, the second on a shorter stream, so the shared workspace's leftovers
 from the first call must not leak into the second.  Inputs beyond the
@@ -46,7 +47,7 @@ def test_static_matches_plan_twice(gpu, batched, threads):
         s = gpu.BatchStream(ops, off, nslots)
         exp = _plan_counters(gpu, a, s, batched, threads)
         got = _static_counters(gpu, a, s, batched, threads)
-        assert got == exp
+        assert got[:gpu.CTR_WAIT_TICKS] == exp[:gpu.CTR_WAIT_TICKS]
         assert got[gpu.CTR_MALLOCS] == s.n_mallocs and got[gpu.CTR_PAIRS] == s.n_mallocs
         assert got[gpu.CTR_BAD] == 0 and got[gpu.CTR_TIMEOUTS] == 0
         assert not torch.any(s.ptrs != 0), "pointer table not empty"
