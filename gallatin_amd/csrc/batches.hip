@@ -106,6 +106,21 @@ __device__ inline void decode_op(const gal_batch_op* ops, uint64_t i, bool activ
   }
 }
 
+// The pointer a published table entry v of `slot` holds: taken out of the
+// table, tag checked; nullptr when the malloc failed.
+__device__ inline void* take_entry(uint64_t* ptrs, uint32_t slot, uint64_t v, op_counts& c) {
+  utils::atomic_store(&ptrs[slot], (uint64_t)0);
+  if (v == kMallocFailed) {
+    c.skipped += 1;
+    return nullptr;
+  }
+  uint64_t* p = reinterpret_cast<uint64_t*>(v);
+  c.bad += utils::atomic_load(p) != (uint64_t)slot;
+  c.bytes += 16;  // tag written at malloc, read here
+  c.pairs += 1;
+  return p;
+}
+
 // The pointer a free op releases: taken out of the table (bounded wait
 // for its malloc to publish it), tag checked; nullptr when the malloc
 // failed or the wait ran out.
@@ -121,16 +136,7 @@ __device__ inline void* wait_free(uint64_t* ptrs, uint32_t slot, op_counts& c,
     v = utils::atomic_load(&ptrs[slot]);
     c.polls += 1;
   }
-  utils::atomic_store(&ptrs[slot], (uint64_t)0);
-  if (v == kMallocFailed) {
-    c.skipped += 1;
-    return nullptr;
-  }
-  uint64_t* p = reinterpret_cast<uint64_t*>(v);
-  c.bad += utils::atomic_load(p) != (uint64_t)slot;
-  c.bytes += 16;  // tag written at malloc, read here
-  c.pairs += 1;
-  return p;
+  return take_entry(ptrs, slot, v, c);
 }
 
 // Decode op i: the pointer to free (taken out of the table, tag checked) or
@@ -343,7 +349,10 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     }
     c.malloc_ticks += wall_clock64() - t0;
     // Frees go through the wave path in every form: a workgroup-wide free
-    // would hold every wave behind the slowest wait.
+    // would hold every wave behind the slowest wait.  (Setting aside frees
+    // whose entry is not yet published, to wait for them after the chunk's
+    // other frees, moved wave time from waiting to freeing and did not
+    // shorten a replay.)
     for (uint32_t r = 0; r < rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
       bool is_free = false;

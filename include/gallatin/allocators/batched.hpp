@@ -16,7 +16,8 @@
 //     other waves.
 //
 // Requests the batch cannot serve (whole blocks, multi-segment runs, runs of
-// 4 KiB slices, or a class that needs more than kRanges claims) go through
+// largest slices in batch_step or across two claims, or a class that needs
+// more than kRanges claims) go through
 // the per-wave path of gallatin.hpp, so results and failure behaviour are
 // those of malloc / free.  In the deterministic mode the class claims take
 // slot 0 and smallest-id segments like the per-wave path, so the slices a
@@ -125,36 +126,61 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
 
 struct malloc_req {
   uint32_t t;     // size class
-  uint32_t rank;  // position among the workgroup's requests of class t
+  uint32_t rank;  // first slice among the workgroup's claims of class t
+  uint32_t cnt;   // slices: 1, or a run of slices of the largest class
   bool batched;   // served by the class claims (else the wave path)
 };
 
-// Phase 1: class and rank of this thread's request.
+// Phase 1: class and rank of this thread's request.  With kRuns, a request
+// above the largest slice that malloc serves as a run of cnt largest slices
+// (gallatin.cuh:1210-1217) joins the largest class's claim with weight cnt;
+// the deterministic mode leaves runs to the wave path.
+template <bool kRuns = false>
 __device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint64_t size,
                                                    lds_state& L) {
-  malloc_req r{0u, 0u, false};
+  malloc_req r{0u, 0u, 1u, false};
   if (want) {
-    r.t = size < s->smallest
-              ? 0u
-              : (uint32_t)(utils::get_first_bit_bigger(size) - (int)s->smallest_bits);
+    const int cl = utils::get_first_bit_bigger(size);
+    r.t = size < s->smallest ? 0u : (uint32_t)(cl - (int)s->smallest_bits);
     r.batched = r.t < s->num_trees;
+    if (kRuns && !r.batched && cl < (int)s->block_tree_bits && !detail::deterministic(s)) {
+      r.cnt = 1u << (r.t - (s->num_trees - 1));
+      r.t = s->num_trees - 1;
+      r.batched = true;
+    }
   }
-  if (r.batched) r.rank = atomicAdd(&L.count[r.t], 1u);
+  if (r.batched) r.rank = atomicAdd(&L.count[r.t], r.cnt);
   return r;
 }
 
 // Phase 3: the byte offset of this thread's slice, or kFail.  The ranges
 // are rewritten only by the next batch's claims, after a barrier every
-// thread reaches after reading them here.
+// thread reaches after reading them here.  A run takes its slices only when
+// they lie in one range, that is in one block, and then credits the cnt - 1
+// frees its one free will not make (block_correct_frees, block.cuh:267-281).
+// A run across two ranges, or past what the claims served, credits back
+// the slices the claims took for it and goes to the wave path.
+template <bool kRuns = false>
 __device__ __forceinline__ uint64_t malloc_take(gal_state* s, const malloc_req& r,
-                                                const lds_state& L) {
+                                                const lds_state& L, uint32_t p) {
   if (!r.batched) return kFail;
-  for (uint32_t k = 0; k < L.nranges[r.t]; ++k) {
+  const uint32_t nr = L.nranges[r.t];
+  for (uint32_t k = 0; k < nr; ++k) {
     const uint32_t f = L.first[r.t][k];
     if (r.rank >= f && r.rank < f + L.len[r.t][k]) {
+      if (kRuns && r.rank + r.cnt > f + L.len[r.t][k]) break;
+      if (kRuns && r.cnt > 1) detail::block_release(s, L.blk[r.t][k], r.cnt - 1, p);
       const uint64_t off = (uint64_t)L.start[r.t][k] + (r.rank - f) +
                            ((uint64_t)L.blk[r.t][k] << 12);
       return detail::offset_to_bytes(s, off, r.t);
+    }
+  }
+  if (kRuns && r.cnt > 1) {
+    for (uint32_t k = 0; k < nr; ++k) {
+      const uint32_t f = L.first[r.t][k], e = f + L.len[r.t][k];
+      const uint32_t lo = r.rank > f ? r.rank : f;
+      const uint32_t hi = r.rank + r.cnt < e ? r.rank + r.cnt : e;
+      if (lo < hi) detail::block_release(s, L.blk[r.t][k], hi - lo, p);
     }
   }
   return kFail;
@@ -225,13 +251,14 @@ __device__ __forceinline__ void free_scan(gal_state* s, lds_state& L, uint32_t m
 // ---- workgroup entry points ----
 
 // Each thread with `want` gets a pointer of `size` bytes (or nullptr).
+// Runs of largest slices are claimed with the classes (see malloc_stage).
 __device__ inline void* batch_malloc(gal_state* s, bool want, uint64_t size, lds_state& L) {
   const uint32_t p = detail::wave_part(s);
-  const malloc_req r = malloc_stage(s, want, size, L);
+  const malloc_req r = malloc_stage<true>(s, want, size, L);
   __syncthreads();
   if (threadIdx.x < 64) claim_classes(s, L, p);
   __syncthreads();
-  const uint64_t bytes = malloc_take(s, r, L);
+  const uint64_t bytes = malloc_take<true>(s, r, L, p);
   void* out = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
   const bool rest = want && bytes == kFail;
   if (__ballot(rest)) {
@@ -275,7 +302,7 @@ __device__ inline void* batch_step(gal_state* s, void* old_ptr, bool want, uint6
   if (__ballot(rest_free)) {
     if (rest_free) detail::free_impl(s, old_ptr);
   }
-  const uint64_t bytes = malloc_take(s, r, L);
+  const uint64_t bytes = malloc_take(s, r, L, p);
   void* out = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
   const bool rest = want && bytes == kFail;
   if (__ballot(rest)) {

@@ -139,3 +139,35 @@ def test_claim_crossing_slice_4095(gpu, name, kind):
     assert got == _oracle_claims(o, sizes)[0], "oracle's coalesced claims"
     _free_all_and_check_fresh(gpu, a, list(offs) + pre)
     a.free_on_device()
+
+
+def test_batch_runs_of_largest_slices(gpu):
+    """Requests in (4 KiB, 32 KiB] are runs of 2-8 consecutive 4 KiB slices
+    of the largest class (gallatin.cuh:1210-1217).  A workgroup batch claims
+    them with the class's single slices; a run that would straddle two of
+    the claim's ranges (two blocks) credits its part back and takes the
+    per-wave path.  Two batches of 1,024 requests, about 4,600 slices each,
+    so claims cross block ends: every run lies in one block (a 16 MiB
+    segment here), no two allocations overlap, singles are slice-aligned,
+    and after all frees the instance drains to fresh, which fails if any
+    pre-credit or hand-back is off by one."""
+    rng = np.random.default_rng(31)
+    a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(POOL, 7, print_info=False)
+    spans, offs = [], []
+    for _ in range(2):
+        sizes = [int(x) for x in np.where(rng.random(1024) < 0.3, 4000,
+                                          rng.integers(4097, 32769, 1024))]
+        got = _run(gpu, a, sizes, "batch", 1024)
+        assert np.all(got != np.uint64(~np.uint64(0))), "a request failed"
+        for sz, off in zip(sizes, got):
+            off = int(off)
+            n = 1 if sz <= 4096 else 1 << (H.clog2(sz) - 12)
+            assert off % 4096 == 0
+            assert off >> 24 == (off + n * 4096 - 1) >> 24, "run across two blocks"
+            spans.append((off, off + n * 4096))
+            offs.append(off)
+    spans.sort()
+    assert all(spans[i][1] <= spans[i + 1][0] for i in range(len(spans) - 1)), "overlap"
+    assert a.info()["errors"] == 0
+    _free_all_and_check_fresh(gpu, a, offs)
+    a.free_on_device()
