@@ -33,6 +33,7 @@ CTR_PAIRS, CTR_MISSES, CTR_BAD, CTR_BYTES, CTR_MALLOCS = 0, 1, 2, 3, 4
 CTR_TIMEOUTS, CTR_FREES_SKIPPED = 5, 6
 CTR_WAIT_TICKS, CTR_WAVE_TICKS, CTR_WAIT_POLLS, CTR_MALLOC_TICKS, CTR_FREE_TICKS = 7, 8, 9, 10, 11
 CTR_COUNT = 12
+GAL_BATCH_TIMED = 0x100  # or-ed into `batched`: the persistent kernel times its waves
 
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
@@ -454,13 +455,17 @@ def _batch_mode(batched):
 class BatchPlan:
     """gal_batch_plan: a stream's launches captured in one hipGraph."""
 
-    def __init__(self, alloc, stream, counters, threads=0, batched=True, workgroups=0):
+    def __init__(self, alloc, stream, counters, threads=0, batched=True, workgroups=0,
+                 timed=False):
+        """timed: the stream kernel also adds its wave time split to
+        counters 7, 8, 10 and 11 (GAL_BATCH_TIMED; slows a replay)."""
         self.stream = stream  # keeps the buffers alive
         self.handle = _p()
         _check(lib().gal_batch_plan_create(
             alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(), stream.nbatches,
             stream.ptrs.data_ptr(), stream.nslots, counters.data_ptr(), threads,
-            _batch_mode(batched), workgroups, ctypes.byref(self.handle)))
+            _batch_mode(batched) | (GAL_BATCH_TIMED if timed else 0), workgroups,
+            ctypes.byref(self.handle)))
 
     def launch(self):
         _check(lib().gal_batch_plan_launch(self.handle, _stream()))
@@ -477,15 +482,17 @@ class BatchPlan:
             pass
 
 
-def run_stream_static(alloc, stream, counters=None, threads=0, batched=True, workgroups=0):
+def run_stream_static(alloc, stream, counters=None, threads=0, batched=True, workgroups=0,
+                      timed=False):
     """gal_run_stream_static: the persistent form of the stream, run once on
     the current stream with the library's static workspace (no device
     allocation)."""
     ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
     _check(lib().gal_run_stream_static(alloc.handle, stream.ops.data_ptr(), stream._offsets_ptr(),
                                        stream.nbatches, stream.ptrs.data_ptr(), stream.nslots,
-                                       ctr.data_ptr(), threads, _batch_mode(batched), workgroups,
-                                       _stream()))
+                                       ctr.data_ptr(), threads,
+                                       _batch_mode(batched) | (GAL_BATCH_TIMED if timed else 0),
+                                       workgroups, _stream()))
     return ctr
 
 

@@ -38,13 +38,16 @@ def biggest_for(segment_bytes):
     return min(segment_bytes // 4096, 4096)
 
 
-def measure(alloc, stream, batched, threads, reps=3, warmup=1):
+def measure(alloc, stream, batched, threads, reps=3, warmup=1, split_reps=3):
     """Time `reps` graph replays of `stream`; returns a result dict.  Its
     `events` are the allocator's event counters (gal_get_stats) that moved
-    during the timed replays, summed over them; `wave_time` splits the
-    stream kernel's wave time into malloc rounds, waits of frees for their
-    entries, the frees themselves and the rest (queue, decode), as shares;
-    `wait_polls` counts those waits' polls per replay."""
+    during the timed replays, summed over them; `wait_polls` counts the
+    polls of frees whose entry was not yet published, per replay.
+    `wave_time` splits the stream kernel's wave time into malloc rounds,
+    waits of frees for their entries, the frees themselves and the rest
+    (queue, decode), as shares, from `split_reps` more replays of a timed
+    plan (the clock reads slow a replay, so the rate comes from the
+    untimed one); 0 skips them."""
     import torch
     ctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
     plan = BatchPlan(alloc, stream, ctr, threads=threads, batched=batched)
@@ -65,6 +68,17 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
     finally:
         plan.destroy()
     c = ctr.cpu().tolist()
+    split = None
+    if split_reps:
+        tctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
+        tplan = BatchPlan(alloc, stream, tctr, threads=threads, batched=batched, timed=True)
+        try:
+            for _ in range(split_reps):
+                tplan.launch()
+            torch.cuda.synchronize()
+        finally:
+            tplan.destroy()
+        split = _wave_time(tctr.cpu().tolist())
     total_ms = evs[0].elapsed_time(evs[-1])
     steps = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(reps))
     ok = (c[CTR_MALLOCS] == reps * stream.n_mallocs and c[CTR_BAD] == 0
@@ -80,7 +94,7 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1):
         "violations": c[CTR_BAD] + c[CTR_TIMEOUTS] + (0 if ok else 1),
         "pairs_per_step": c[CTR_PAIRS] / reps,
         "events": {k: s1[k] - s0[k] for k in s1 if s1[k] != s0[k]},
-        "wave_time": _wave_time(c),
+        "wave_time": split,
         "wait_polls": c[CTR_WAIT_POLLS] / reps,
     }
 

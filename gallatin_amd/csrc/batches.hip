@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
                                                      const stream_chunk* chunks, uint32_t nchunks,
                                                      uint32_t* queue, uint64_t* ptrs,
                                                      uint64_t nslots, uint64_t* ctr,
-                                                     uint32_t spin_cap) {
+                                                     uint32_t spin_cap, bool timed) {
   __shared__ batch::lds_state lds;
   __shared__ cache::slice_cache sc;
   __shared__ uint32_t cur;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     // them round by round instead (malloc, then wait and free, per round)
     // held a chunk's later mallocs behind its own earlier waits, so waits
     // chained from chunk to chunk through the stream.
-    const uint64_t t0 = wall_clock64();
+    const uint64_t t0 = timed ? wall_clock64() : 0;
     for (uint32_t r = 0; r < rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
       bool want, is_free;
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
         np = detail::malloc_impl(s, size);
       give_op(np, want, slot, ptrs, c);
     }
-    c.malloc_ticks += wall_clock64() - t0;
+    if (timed) c.malloc_ticks += wall_clock64() - t0;
     // Frees go through the wave path in every form: a workgroup-wide free
     // would hold every wave behind the slowest wait.  (Setting aside frees
     // whose entry is not yet published, to wait for them after the chunk's
@@ -362,16 +362,18 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
         is_free = op.kind == 1 && op.slot < nslots;  // invalid ops were counted above
         slot = op.slot;
       }
-      const uint64_t t1 = wall_clock64();
+      const uint64_t t1 = timed ? wall_clock64() : 0;
       void* old = is_free ? wait_free(ptrs, slot, c, spin_cap, ctr) : nullptr;
-      const uint64_t t2 = wall_clock64();
+      const uint64_t t2 = timed ? wall_clock64() : 0;
       if (old) detail::free_impl(s, old);
-      c.wait_ticks += t2 - t1;
-      c.free_ticks += wall_clock64() - t2;
+      if (timed) {
+        c.wait_ticks += t2 - t1;
+        c.free_ticks += wall_clock64() - t2;
+      }
     }
   }
   if (kMode == 2) cache::cache_flush(s, sc);
-  flush_counts(ctr, c, true);
+  flush_counts(ctr, c, timed);
 }
 
 __global__ void k_stream_reset(uint32_t* queue) {
@@ -491,6 +493,8 @@ static int enqueue_persistent(gal_state* s, const stream_chunk* chunks, uint32_t
                               const gal_batch_op* ops, uint32_t nchunks, uint64_t* ptrs,
                               uint64_t nslots, uint64_t* ctr, uint32_t threads, uint32_t batched,
                               uint32_t workgroups, hipStream_t st) {
+  const bool timed = (batched & GAL_BATCH_TIMED) != 0;
+  batched &= ~GAL_BATCH_TIMED;
   k_stream_reset<<<1, 64, 0, st>>>(queue);
   if (nchunks) {
 #define GAL_STREAM_LAUNCH(T, B)                                                              \
@@ -498,7 +502,7 @@ static int enqueue_persistent(gal_state* s, const stream_chunk* chunks, uint32_t
     unsigned g = fill_grid(k_stream<T, B>, T, workgroups);                                    \
     if (g > nchunks) g = nchunks;                                                             \
     k_stream<T, B><<<g, T, 0, st>>>(s, ops, chunks, nchunks, queue, ptrs, nslots, ctr,       \
-                                    kStreamSpinCap);                                          \
+                                    kStreamSpinCap, timed);                                   \
   } while (0)
     if (batched == 0)
       GAL_STREAM_LAUNCH(256, 0);
@@ -613,8 +617,9 @@ static int check_plan_args(gal_allocator* a, const gal_batch_op* ops, const uint
                            uint32_t nbatches, uint64_t* ptrs, uint64_t nslots, uint64_t* ctr,
                            uint32_t* threads, uint32_t batched) {
   if (!a || !offsets || !ctr || (nslots && !ptrs)) return fail(GAL_ERR_INVALID, "null argument");
-  if (batched > 2)
-    return fail(GAL_ERR_INVALID, "batched is 0 (per thread), 1 (workgroup batches) or 2 (slice cache)");
+  if ((batched & ~GAL_BATCH_TIMED) > 2)
+    return fail(GAL_ERR_INVALID, "batched is 0 (per thread), 1 (workgroup batches) or 2 (slice "
+                                 "cache), with GAL_BATCH_TIMED or-ed in for the timed kernel");
   if (nbatches && offsets[nbatches] && !ops) return fail(GAL_ERR_INVALID, "null ops");
   for (uint32_t b = 0; b < nbatches; ++b)
     if (offsets[b + 1] < offsets[b]) return fail(GAL_ERR_INVALID, "offsets must not decrease");
@@ -676,8 +681,9 @@ int gal_run_batches(gal_allocator* a, const gal_batch_op* ops, const uint64_t* o
                     uint32_t threads, uint32_t batched, uint32_t workgroups, void* stream) {
   const int rc = check_plan_args(a, ops, offsets, nbatches, ptrs, nslots, ctr, &threads, batched);
   if (rc != GAL_OK) return rc;
-  return enqueue_stream(a, ops, offsets, nbatches, ptrs, nslots, ctr, threads, batched,
-                        workgroups, (hipStream_t)stream);
+  // one launch per batch: GAL_BATCH_TIMED times only the persistent kernel
+  return enqueue_stream(a, ops, offsets, nbatches, ptrs, nslots, ctr, threads,
+                        batched & ~GAL_BATCH_TIMED, workgroups, (hipStream_t)stream);
 }
 
 int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint64_t* offsets,
@@ -701,7 +707,7 @@ int gal_batch_plan_create(gal_allocator* a, const gal_batch_op* ops, const uint6
     if (rc != GAL_OK) return rc;
   }
   // chunk table: at most kChunkRounds * threads ops, never across batches
-  const uint32_t per = kChunkRounds * (batched ? threads : 256u);
+  const uint32_t per = kChunkRounds * ((batched & ~GAL_BATCH_TIMED) ? threads : 256u);
   std::vector<stream_chunk> chunks;
   for (uint32_t b = 0; b < nbatches; ++b) {
     for (uint64_t at = offsets[b]; at < offsets[b + 1]; at += per) {
@@ -755,7 +761,7 @@ int gal_run_stream_static(gal_allocator* a, const gal_batch_op* ops, const uint6
   if (nbatches > kStaticMaxBatches)
     return fail(GAL_ERR_INVALID, "the static workspace holds at most 65535 batches; got " +
                                      std::to_string(nbatches));
-  const uint32_t per = kChunkRounds * (batched ? threads : 256u);
+  const uint32_t per = kChunkRounds * ((batched & ~GAL_BATCH_TIMED) ? threads : 256u);
   uint64_t nchunks = 0;
   for (uint32_t b = 0; b < nbatches; ++b) nchunks += (offsets[b + 1] - offsets[b] + per - 1) / per;
   if (nchunks > kStaticMaxChunks)

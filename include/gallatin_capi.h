@@ -207,10 +207,11 @@ int gal_run_wave_stream(gal_allocator *a, const gal_wave_step *steps_dev,
  * [5] bounded waits that ran out (batched streams), [6] frees of a table
  * entry that held nullptr (batched streams: the malloc had failed).
  * Batched streams also count [9] polls of frees whose entry was not yet
- * published.  The persistent stream kernel adds, per wave, wall-clock ticks
- * (wall_clock64) spent [7] waiting for entries, [10] in its malloc rounds
- * and [11] freeing, and [8] its lifetime, so [7] / [8] is the share of the
- * kernel's wave time spent waiting. */
+ * published.  The persistent stream kernel, when timed (GAL_BATCH_TIMED),
+ * adds per wave wall-clock ticks (wall_clock64) spent [7] waiting for
+ * entries, [10] in its malloc rounds and [11] freeing, and [8] its
+ * lifetime, so [7] / [8] is the share of the kernel's wave time spent
+ * waiting. */
 enum { GAL_CTR_PAIRS = 0, GAL_CTR_MISSES = 1, GAL_CTR_BAD = 2,
        GAL_CTR_BYTES = 3, GAL_CTR_MALLOCS = 4, GAL_CTR_TIMEOUTS = 5,
        GAL_CTR_FREES_SKIPPED = 6, GAL_CTR_WAIT_TICKS = 7, GAL_CTR_WAVE_TICKS = 8,
@@ -463,6 +464,12 @@ int gal_check_batch_stream(const gal_batch_op *ops_host, const uint64_t *offsets
  * host once and refuses a stream that breaks this (gal_check_batch_stream
  * with persistent = 1) with GAL_ERR_INVALID.  The buffers must stay valid
  * until the plan is destroyed. */
+/* Or-ed into `batched` of the persistent forms (gal_batch_plan_create,
+ * gal_run_stream_static): the stream kernel reads the wall clock around its
+ * phases and adds counters [7], [8], [10] and [11].  The reads cost about
+ * 7 % of a replay, so untimed runs leave those counters at 0.  The eager
+ * gal_run_batches ignores it. */
+#define GAL_BATCH_TIMED 0x100u
 typedef struct gal_batch_plan gal_batch_plan;
 int gal_batch_plan_create(gal_allocator *a, const gal_batch_op *ops_dev,
                           const uint64_t *offsets, uint32_t nbatches,

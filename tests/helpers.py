@@ -239,6 +239,28 @@ def log_uniform_mean_bytes():
     return 4095 / math.log(4096)
 
 
+def check_tree_parents(layers):
+    """A quiescent tree's invariant (veb.hpp sync_up): bit i of each word
+    of a layer is set exactly when word i of the layer below is non-zero,
+    and no bit points past the layer below.  Layers top first."""
+    for up, down in zip(layers[:-1], layers[1:]):
+        idx = np.arange(len(down))
+        bit = (up[idx // 64] >> (idx % 64).astype(np.uint64)) & np.uint64(1)
+        bad = np.nonzero((bit != 0) != (down != 0))[0]
+        assert len(bad) == 0, f"parent bits disagree with child words {bad[:8].tolist()}"
+        extra = len(up) * 64 - len(down)
+        if extra:
+            tail = up[-1] >> np.uint64(64 - extra) if extra < 64 else up[-1]
+            assert tail == 0, "parent bits past the last child word"
+
+
+def check_all_tree_parents(a):
+    check_tree_parents(a.tree_layers(-1))
+    for t in range(a.num_trees):
+        for p in range(a.partitions):
+            check_tree_parents(a.tree_layers(t, p))
+
+
 def check_fresh(a):
     """Metadata of a drained instance (nothing allocated, nothing pinned)
     equals the freshly initialised state bit for bit."""

@@ -294,9 +294,12 @@ def test_block_and_multi_segment_config(gpu):
     [64 KiB, 64 MiB] (whole blocks and 2-4 segment runs) and 64 requests
     uniform in [256 MiB, 2 GiB] (16-128 segment runs) in one launch.  Demand
     is far above the pool, so many return nullptr; the served ones are in
-    the pool, aligned and disjoint and fill most of it.  All are freed in
-    random order, and after a drain every bitmap, chunk id and counter is
-    the fresh state bit for bit."""
+    the pool, aligned and disjoint and fill most of it.  The wave's
+    multi-segment claims race block requests for segments, so some gathers
+    lose a claim and put the bits back: right after the launch, and again
+    after the frees, every tree's parent bits must match its child words.
+    All are freed in random order, and after a drain every bitmap, chunk id
+    and counter is the fresh state bit for bit."""
     import torch
     rng = np.random.default_rng(33)
     sizes = np.concatenate([rng.integers(64 << 10, (64 << 20) + 1, 4096),
@@ -312,10 +315,12 @@ def test_block_and_multi_segment_config(gpu):
     span, _ = H.reserved_vec(sizes)
     assert int(span[served].sum()) <= pool
     assert int(span[served].sum()) > pool // 2, "most of the pool should be in use"
+    H.check_all_tree_parents(a)
     live = ptrs[torch.from_numpy(np.nonzero(served)[0]).cuda()]
     perm = torch.from_numpy(rng.permutation(len(live))).cuda()
     a.free_bulk(live[perm].contiguous())
     torch.cuda.synchronize()
+    H.check_all_tree_parents(a)
     a.drain()
     check_fresh(a)
     a.free_on_device()

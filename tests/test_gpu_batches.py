@@ -86,3 +86,30 @@ def test_autotune_sweeps_both_batched_forms(gpu, small_stream):
     assert {r["form"] for r in table} == {"workgroup", "cache", "per-thread"}
     assert all(r["violations"] == 0 for r in table), table
     assert best is not None and best["path"] == "batched"
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_timed_plan_splits_wave_time(gpu, small_stream, batched):
+    """A plan made with timed=True (GAL_BATCH_TIMED) serves the stream the
+    same way and adds the stream kernel's wave time split: malloc rounds,
+    waits and frees are parts of the waves' lifetime.  An untimed plan
+    leaves those counters at zero."""
+    import torch
+    a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(8 << 30, 8, print_info=False)
+    for timed in (False, True):
+        ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
+        plan = gpu.BatchPlan(a, small_stream, ctr, batched=batched, timed=timed)
+        plan.launch()
+        torch.cuda.synchronize()
+        plan.destroy()
+        c = ctr.cpu().tolist()
+        _check(gpu, c, small_stream)
+        parts = [c[gpu.CTR_MALLOC_TICKS], c[gpu.CTR_WAIT_TICKS], c[gpu.CTR_FREE_TICKS]]
+        if not timed:
+            assert parts == [0, 0, 0] and c[gpu.CTR_WAVE_TICKS] == 0
+        else:
+            assert c[gpu.CTR_WAVE_TICKS] > 0 and parts[0] > 0
+            assert sum(parts) <= c[gpu.CTR_WAVE_TICKS]
+    a.drain()
+    H.check_fresh(a)
+    a.free_on_device()
