@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
                                                      uint32_t spin_cap, bool timed) {
   __shared__ batch::lds_state lds;
   __shared__ cache::slice_cache sc;
+  __shared__ uint32_t ranks[kMode == 1 ? kChunkRounds * kThreads : 1];  // per op of a chunk
   __shared__ uint32_t cur;
   if (kMode == 1) batch::batch_init(lds);
   if (kMode == 2) cache::cache_init(sc);
@@ -332,20 +333,61 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     // held a chunk's later mallocs behind its own earlier waits, so waits
     // chained from chunk to chunk through the stream.
     const uint64_t t0 = timed ? wall_clock64() : 0;
-    for (uint32_t r = 0; r < rounds; ++r) {
-      const uint32_t j = r * kThreads + threadIdx.x;
-      bool want, is_free;
-      uint64_t size;
-      uint32_t slot;
-      decode_op(ops, ch.begin + j, j < ch.count, nslots, c, want, size, slot, is_free);
-      void* np = nullptr;
-      if (kMode == 1)
-        np = batch::batch_malloc(s, want, size, lds);
-      else if (kMode == 2)
-        np = cache::cache_malloc(s, sc, want, size);
-      else if (want)
-        np = detail::malloc_impl(s, size);
-      give_op(np, want, slot, ptrs, c);
+    if constexpr (kMode == 1) {
+      // The workgroup form counts the requests of all the chunk's rounds
+      // first and claims them together: two barriers and one claim per
+      // class for the chunk, where one batch_malloc per round took two
+      // barriers and one claim per class each.
+      const uint32_t p = detail::wave_part(s);
+      for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t j = r * kThreads + threadIdx.x;
+        bool want, is_free;
+        uint64_t size;
+        uint32_t slot;
+        decode_op(ops, ch.begin + j, j < ch.count, nslots, c, want, size, slot, is_free);
+        ranks[j] = batch::malloc_stage<true>(s, want, size, lds).rank;
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) batch::claim_classes(s, lds, p);
+      __syncthreads();
+      for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t j = r * kThreads + threadIdx.x;
+        bool want = false;
+        uint64_t size = 0;
+        uint32_t slot = 0;
+        if (j < ch.count) {  // decoded (and invalid ops counted) above
+          const gal_batch_op op = ops[ch.begin + j];
+          want = op.kind == 0 && op.slot < nslots;
+          size = op.arg;
+          slot = op.slot;
+        }
+        batch::malloc_req q = batch::malloc_classify<true>(s, want, size);
+        q.rank = ranks[j];
+        const uint64_t bytes = batch::malloc_take<true>(s, q, lds, p);
+        void* np = bytes == kFail ? nullptr : (void*)(s->memory + bytes);
+        const bool rest = want && bytes == kFail;
+        if (__ballot(rest)) {
+          void* w = nullptr;
+          if (rest) w = detail::malloc_impl(s, size);
+          if (rest) np = w;
+        }
+        if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
+        give_op(np, want, slot, ptrs, c);
+      }
+    } else {
+      for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t j = r * kThreads + threadIdx.x;
+        bool want, is_free;
+        uint64_t size;
+        uint32_t slot;
+        decode_op(ops, ch.begin + j, j < ch.count, nslots, c, want, size, slot, is_free);
+        void* np = nullptr;
+        if (kMode == 2)
+          np = cache::cache_malloc(s, sc, want, size);
+        else if (want)
+          np = detail::malloc_impl(s, size);
+        give_op(np, want, slot, ptrs, c);
+      }
     }
     if (timed) c.malloc_ticks += wall_clock64() - t0;
     // Frees go through the wave path in every form: a workgroup-wide free

@@ -131,13 +131,12 @@ struct malloc_req {
   bool batched;   // served by the class claims (else the wave path)
 };
 
-// Phase 1: class and rank of this thread's request.  With kRuns, a request
+// Class and weight of a request, without its rank.  With kRuns, a request
 // above the largest slice that malloc serves as a run of cnt largest slices
 // (gallatin.cuh:1210-1217) joins the largest class's claim with weight cnt;
 // the deterministic mode leaves runs to the wave path.
 template <bool kRuns = false>
-__device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint64_t size,
-                                                   lds_state& L) {
+__device__ __forceinline__ malloc_req malloc_classify(gal_state* s, bool want, uint64_t size) {
   malloc_req r{0u, 0u, 1u, false};
   if (want) {
     const int cl = utils::get_first_bit_bigger(size);
@@ -149,6 +148,15 @@ __device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint
       r.batched = true;
     }
   }
+  return r;
+}
+
+// Phase 1: class and rank of this thread's request (malloc_classify, then
+// its place in the workgroup's count of its class).
+template <bool kRuns = false>
+__device__ __forceinline__ malloc_req malloc_stage(gal_state* s, bool want, uint64_t size,
+                                                   lds_state& L) {
+  malloc_req r = malloc_classify<kRuns>(s, want, size);
   if (r.batched) r.rank = atomicAdd(&L.count[r.t], r.cnt);
   return r;
 }
