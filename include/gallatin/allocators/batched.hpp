@@ -3,9 +3,9 @@
 //
 //   malloc: requests are counted per size class in LDS; one wave then claims
 //     every class's whole count at once, lane c for class c, with one
-//     counter atomic per class (more only when a claim runs past slice 4095
-//     and the block is replaced), and each request takes its slice by its
-//     rank in the class.  This is the reference's coalesced group claim
+//     counter compare-and-swap per class (more only when it races another
+//     claim or a claim runs past slice 4095 and the block is replaced), and
+//     each request takes its slice by its rank in the class.  This is the reference's coalesced group claim
 //     (gallatin.cuh:865-1078, block.cuh:245-264) with the group widened from
 //     a warp to a workgroup.
 //   free: requests are grouped per block in an LDS hash table and each block
@@ -87,14 +87,30 @@ __device__ inline void claim_classes(gal_state* s, lds_state& L, uint32_t p) {
     const uint32_t slot = (uint32_t)(sb >> 32), blk = (uint32_t)sb;
     const bool have = open && blk < kRefilling;
     const bool waiting = open && blk == kRefilling;  // another wave refills
-    uint32_t old = 0;
-    if (have) old = utils::atomic_add(&detail::block_at(s, blk)->malloc_counter, need - served);
-    const uint32_t cnt = old & kCountMask;
-    const bool tag_ok = have && (old >> kTreeShift) == block_tag(c, p);
-    uint32_t take = 0;
-    if (have && cnt < 4096u) take = min(need - served, 4096u - cnt);
-    if (have && !tag_ok && take) detail::block_release(s, blk, take, p);  // stale block
-    const bool got = tag_ok && take > 0;
+    // The claim takes exactly what the block has left, by compare-and-swap,
+    // and never adds to a full or stale block.  A workgroup's demand can be
+    // many blocks' worth (runs of largest slices over a whole chunk), and
+    // adding it to a full block that is about to be replaced, once per
+    // retrying workgroup, would carry the 20-bit count into the tag.
+    uint32_t cnt = 0, take = 0;
+    bool tag_ok = false;
+    if (have) {
+      uint32_t* mc = &detail::block_at(s, blk)->malloc_counter;
+      uint32_t cur = utils::atomic_load(mc);
+      for (uint32_t g = 0; g < 64; ++g) {
+        cnt = cur & kCountMask;
+        tag_ok = (cur >> kTreeShift) == block_tag(c, p);
+        if (!tag_ok || cnt >= 4096u) break;
+        const uint32_t want = min(need - served, 4096u - cnt);
+        const uint32_t seen = utils::atomic_cas(mc, cur, cur + want);
+        if (seen == cur) {
+          take = want;
+          break;
+        }
+        cur = seen;
+      }
+    }
+    const bool got = take > 0;
     if (got) {
       L.blk[c][nr] = blk;
       L.start[c][nr] = cnt;

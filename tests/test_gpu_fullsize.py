@@ -84,3 +84,35 @@ def test_batched_zipf_full_size(gpu, zipf_full, batched, threads):
     a.drain()
     H.check_fresh(a)
     a.free_on_device()
+
+
+@pytest.mark.parametrize("batched", [1, 0])
+def test_batched_zipf_small_segments(gpu, zipf_full, batched):
+    """The full stream at 4 MiB segments and 256 B smallest slices, a point
+    of the autotuner's grid: the largest class has 1 KiB slices and one
+    block per segment, and a third of the requests are runs of up to 512 of
+    them, so a workgroup's claims ask for many blocks' worth while blocks
+    fill and are replaced all the time.  Claims that added their whole
+    demand to a full block once carried its count into the tag here
+    (allocator errors).  No allocator error, no bad tag, every successful
+    malloc freed once, and fresh after the drain."""
+    import torch
+    a = gpu.Gallatin(4 << 20, 256, 1024).generate_on_device(
+        64 << 30, SEED, print_info=False, slots_min=32, flags=gpu.GAL_FLAG_NO_PAYLOAD_FENCE)
+    ctr = torch.zeros(gpu.CTR_COUNT, dtype=torch.int64, device="cuda")
+    plan = gpu.BatchPlan(a, zipf_full, ctr, threads=1024 if batched else 256, batched=batched)
+    for rep in range(3):
+        ctr.zero_()
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        c = ctr.cpu().tolist()
+        assert c[gpu.CTR_MALLOCS] == zipf_full.n_mallocs, rep
+        assert c[gpu.CTR_BAD] == 0 and c[gpu.CTR_TIMEOUTS] == 0, rep
+        assert c[gpu.CTR_PAIRS] == c[gpu.CTR_MALLOCS] - c[gpu.CTR_MISSES], rep
+        assert not torch.any(zipf_full.ptrs != 0), f"replay {rep}: table not empty"
+        assert a.info()["errors"] == 0, (rep, a.stats())
+    plan.destroy()
+    a.drain()
+    H.check_fresh(a)
+    a.free_on_device()
