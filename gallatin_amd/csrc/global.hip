@@ -809,23 +809,23 @@ __global__ void k_serve_init(serving_session* ss, uint32_t* free_slots, int32_t*
 }
 
 // Words [at, stop) of session `sid` into chunk memory `w` that starts at
-// position cs: every lane of the wave writes 16-byte pairs (chunks are at
-// least 128-byte aligned and start at even positions).
+// position cs: the `width` lanes of a group (this lane is number `sl` in
+// it) write 16-byte pairs (chunks are at least 128-byte aligned and start at
+// even positions).
 __device__ __forceinline__ void write_words(uint64_t* w, uint64_t cs, uint64_t at, uint64_t stop,
-                                            uint64_t sid) {
-  const uint32_t lane = gallatin::wave::lane();
+                                            uint64_t sid, uint32_t sl, uint32_t width) {
   const uint64_t tag = sid << 32;
   if (at & 1) {  // odd head word
-    if (lane == 0) w[at - cs] = tag | at;
+    if (sl == 0) w[at - cs] = tag | at;
     ++at;
   }
   const uint64_t pairs = (stop - at) >> 1;
   ulonglong2* v = reinterpret_cast<ulonglong2*>(w + (at - cs));
-  for (uint64_t j = lane; j < pairs; j += 64) {
+  for (uint64_t j = sl; j < pairs; j += width) {
     const uint64_t p = at + 2 * j;
     v[j] = make_ulonglong2(tag | p, tag | (p + 1));
   }
-  if ((stop - at) & 1 && lane == 0) w[stop - 1 - cs] = tag | (stop - 1);
+  if ((stop - at) & 1 && sl == 0) w[stop - 1 - cs] = tag | (stop - 1);
 }
 
 // Session pass: each wave takes 64 session slots at a time.
@@ -854,11 +854,16 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
     int32_t at0 = 0;
     if (em && gallatin::wave::is_first()) at0 = atomicAdd(top, (int32_t)gallatin::wave::count(em));
     at0 = (int32_t)gallatin::wave::readfirst((uint32_t)at0);
-    if (ends) {  // free the vector, return the slot
-      serving_session& se = ss[slot];
-      freed += se.vec.chunks_held();
-      se.vec.free_all();
-      se.alive = 0;
+    // ending sessions' vectors are freed by the whole wave, one at a time
+    for (uint64_t m = em; m; m &= m - 1) {
+      const int l = gallatin::wave::first(m);
+      session_vec* ev = reinterpret_cast<session_vec*>(
+          gallatin::wave::readlane((uint64_t)&ss[slot].vec, l));
+      const uint32_t nf = ev->free_all_wave();
+      if ((int)lane == l) freed += nf;
+    }
+    if (ends) {  // return the slot
+      ss[slot].alive = 0;
       free_slots[at0 + (int32_t)gallatin::wave::count(em & gallatin::wave::lanes_below())] = slot;
       ++ended;
     }
@@ -900,22 +905,29 @@ __global__ __launch_bounds__(256) void k_serve_sessions(serving_session* ss, uin
       }
     }
     if (live) refused += at + want - end;
-    // appends, one session at a time, all lanes writing
-    for (uint64_t todo = __ballot(live && end > at); todo; todo &= todo - 1) {
-      const int l = gallatin::wave::first(todo);
-      session_vec* sv = reinterpret_cast<session_vec*>(gallatin::wave::readlane((uint64_t)v, l));
-      const uint64_t sid = gallatin::wave::readlane(id, l);
-      uint64_t a = gallatin::wave::readlane(at, l);
-      const uint64_t e = gallatin::wave::readlane(end, l);
+    // appends, four sessions at a time, each written by a group of 16 lanes
+    const uint32_t grp = lane >> 4, sl = lane & 15;
+    for (uint64_t todo = __ballot(live && end > at); todo;) {
+      int pick = -1;
+      for (uint32_t k = 0; k < 4 && todo; ++k, todo &= todo - 1)
+        if (k == grp) pick = gallatin::wave::first(todo);
+      // every lane takes part in the fetches (a lane reads a masked lane's
+      // value as undefined); a group without a session copies its own lane
+      const int src = pick < 0 ? (int)lane : pick;
+      session_vec* sv = reinterpret_cast<session_vec*>(gallatin::wave::fetch((uint64_t)v, src));
+      const uint64_t sid = gallatin::wave::fetch(id, src);
+      uint64_t a = gallatin::wave::fetch(at, src);
+      const uint64_t e_src = gallatin::wave::fetch(end, src);
+      const uint64_t e = pick < 0 ? a : e_src;
       while (a < e) {
         const uint32_t c = session_vec::chunk_of(a);
         const uint64_t cs = session_vec::chunk_start(c);
         const uint64_t ce = cs + session_vec::chunk_items(c);
         const uint64_t stop = ce < e ? ce : e;
-        write_words(sv->chunk(c), cs, a, stop, sid);
+        write_words(sv->chunk(c), cs, a, stop, sid, sl, 16);
         a = stop;
       }
-      if (lane == 0) sv->size = e;
+      if (pick >= 0 && sl == 0) sv->size = e;
     }
     if (live) words += end - at;
   }

@@ -118,6 +118,36 @@ struct chunked_vector {
     }
     size = 0;
   }
+
+  // free_all by a whole wave: every active lane calls it for the same
+  // vector, and lane i frees chunks i, i + 64, ... in one global_free call
+  // of the wave per 64 chunks, where free_all makes one call per chunk.
+  // Returns the chunks freed (the index not counted), on every lane.
+  __device__ __noinline__ uint32_t free_all_wave() {
+    static_assert(n_grow <= 64, "one lane per doubling chunk");
+    const uint32_t lane = __lane_id();
+    T* g = lane < n_grow ? grow[lane] : nullptr;
+    uint32_t n = (uint32_t)__builtin_popcountll(__ballot(g != nullptr));
+    if (g) gallatin::allocators::global_free(g);
+    T** t = tail;
+    if (t) {
+      // tail chunks are installed in order, so the first empty entry ends them
+      for (uint32_t base = 0; base < tail_chunks; base += 64) {
+        const bool in = base + lane < tail_chunks;
+        T* c = in ? t[base + lane] : nullptr;
+        n += (uint32_t)__builtin_popcountll(__ballot(c != nullptr));
+        if (c) gallatin::allocators::global_free(c);
+        if (__ballot(in && c == nullptr)) break;
+      }
+      if (lane == (uint32_t)__builtin_ctzll(__ballot(1))) gallatin::allocators::global_free(t);
+    }
+    if (lane < n_grow) grow[lane] = nullptr;
+    if (lane == 0) {
+      tail = nullptr;
+      size = 0;
+    }
+    return n;
+  }
 };
 
 }  // namespace data_structs
