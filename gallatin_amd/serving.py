@@ -84,6 +84,7 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
     reserve = int(num_segments * reserve_fraction)
     sv = Serving(max_sessions, seed)
     ctr = torch.zeros(16, dtype=torch.int64, device="cuda")
+    events0 = global_allocator().stats()
     try:
         trace = []
         for k in range(warmup):
@@ -112,6 +113,7 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
         tab = sv.table()
         segments_free = global_allocator().info()["segments_free"]
         gate_end = sv.gate()
+        events1 = global_allocator().stats()
     finally:
         sv.destroy()
     timed = [c[i] - warm[i] for i in range(16)]
@@ -142,4 +144,7 @@ def run(torch, steps, warmup, seed=42, max_sessions=1 << 19, arrivals_mean=2000,
             ("admitted", "refused", "ended", "words", "words_refused", "chunks",
              "chunk_failed", "chunks_freed", "ht_inserted", "ht_removed", "ht_absent",
              "ht_failed"))},
+        # the global allocator's event counters (gal_get_stats) over the run
+        "allocator_events": {k: events1[k] - events0[k] for k in events1
+                             if events1[k] != events0[k]},
     }

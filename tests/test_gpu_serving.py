@@ -111,22 +111,27 @@ def test_serving_out_of_memory_fails_fast(gpu):
         gpu.free_global_allocator()
 
 
-def test_serving_full_config(gpu):
+@pytest.mark.parametrize("seed", [1234, 1238, 1239, 1240])
+def test_serving_full_config(gpu, seed):
     """The serving configuration at full size: 200 GiB pool, Poisson(2000)
     arrivals, geometric(0.01) lifetimes, 2^20 table operations per step on
     keys in [1, 2^28], a 10 % free-segment reserve.  Over 700 steps the pool
     fills, the gate closes (arrivals refused) and opens again; no append is
-    cut short, every word reads back, and teardown drains to fresh."""
+    cut short, every word reads back, and teardown drains to fresh.  Several
+    seeds: block claims that livelocked on a segment's last blocks (dozens
+    of waves subtracting and adding back on one count) failed chunk
+    allocations in about a third of the runs, at any seed."""
     import torch
     from gallatin_amd import serving as SV
-    gpu.init_global_allocator(200 << 30, 1234, print_info=False)
+    gpu.init_global_allocator(200 << 30, seed, print_info=False)
     try:
-        r = SV.run(torch, steps=50, warmup=650, seed=1234, max_sessions=1 << 19,
+        r = SV.run(torch, steps=50, warmup=650, seed=seed, max_sessions=1 << 19,
                    arrivals_mean=2000, p_end=0.01, ht_ops=1 << 20, reserve_fraction=0.10,
                    trace_every=50)
         c = r["counters"]
         _check_counts(r, 700, 1 << 20)
-        assert c["words_refused"] == 0 and c["chunk_failed"] == 0 and c["ht_failed"] == 0
+        assert c["words_refused"] == 0 and c["chunk_failed"] == 0 and c["ht_failed"] == 0, \
+            (c, r["allocator_events"], r["trace"])
         assert c["refused"] > 0, "the admission gate never closed"
         assert min(t["segments_free"] for t in r["trace"]) >= r["reserve_segments"] // 2
         g = gpu.global_allocator()
