@@ -303,15 +303,19 @@ __device__ inline void retire_block(gal_state* s, uint32_t blk, uint32_t my_p) {
 // memory_table.cuh:915-993 get_block for a segment of class t, home p.  The
 // packed segment word gives the active count and the owner in one atomic.
 // Returns the block id, with bit 32 set when it was the segment's last.
-__device__ inline uint64_t get_block(gal_state* s, uint64_t seg, uint32_t t, uint32_t p) {
-  // Look before taking: a claim subtracts from the count and adds back when
-  // it found none, so with many waves on a segment that has one or two free
-  // blocks left every claimer can see the others' transient subtractions
-  // and fail, on and on.  A claimer that sees no free block (or another
-  // owner) leaves without touching the count.
-  const uint64_t cur = atomic_load(&s->seg_state[seg]);
-  if ((int32_t)(cur >> 32) < 0 || (uint32_t)(cur & 0xFFFF) != t || seg_home(cur) != p)
-    return kNullBlock;
+__device__ inline uint64_t get_block(gal_state* s, uint64_t seg, uint32_t t, uint32_t p,
+                                    bool look) {
+  // With `look`, look before taking: a claim subtracts from the count and
+  // adds back when it found none, so with many waves retrying on a segment
+  // that has one or two free blocks left every claimer can see the others'
+  // transient subtractions and fail, on and on.  A retrying claimer that
+  // sees no free block (or another owner) leaves without touching the
+  // count.  A first attempt takes directly, which saves the load.
+  if (look) {
+    const uint64_t cur = atomic_load(&s->seg_state[seg]);
+    if ((int32_t)(cur >> 32) < 0 || (uint32_t)(cur & 0xFFFF) != t || seg_home(cur) != p)
+      return kNullBlock;
+  }
   const uint64_t old = atomic_sub(&s->seg_state[seg], kActiveOne);
   const int32_t ac = (int32_t)(old >> 32);
   if (ac < 0 || (uint32_t)(old & 0xFFFF) != t || seg_home(old) != p) {
@@ -450,18 +454,13 @@ __device__ inline uint64_t gather_new_segment(gal_state* s, uint32_t t, uint32_t
 // Segment of class t (partition p) to take a block from: smallest id in
 // deterministic mode (find_first_valid_index, veb.cuh:940-944), else from a
 // hashed start (find_random_valid_index, veb.cuh:946-977) wrapping to 0.
-// `from`, when not kFail, replaces the hashed start (a wave whose claim on
-// a segment failed searches on from the next one).
-__device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bool det,
-                                        uint64_t from = kFail) {
+__device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bool det) {
   const gal_tree& tr = class_tree(s, t, p);
   if (det) return veb::find_ge(tr, 0);
-  uint64_t start = from;
-  if (start == kFail || start >= tr.universe) {
-    const uint64_t gid = wave::readfirst(
-        (uint64_t)threadIdx.x + (uint64_t)blockIdx.x * blockDim.x);
-    start = hashers::MurmurHash64A_u64(gid, s->seed + t + 1) % tr.universe;
-  }
+  const uint64_t gid = wave::readfirst(
+      (uint64_t)threadIdx.x + (uint64_t)blockIdx.x * blockDim.x);
+  const uint64_t start =
+      hashers::MurmurHash64A_u64(gid, s->seed + t + 1) % tr.universe;
   uint64_t x = veb::find_ge(tr, start);
   if (x == kFail && start != 0) x = veb::find_ge(tr, 0);
   return x;
@@ -474,11 +473,10 @@ __device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bo
 __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t p,
                                              bool det, uint32_t polls) {
   int attempts = 0;
-  uint64_t from = kFail;  // after a failed claim: where the next search starts
   const uint32_t lk = tree_lock(s, t, p);
   for (uint32_t spins = 0; attempts < (int)polls && spins < kSpinCap;
        ++spins) {
-    uint64_t seg = find_segment(s, t, p, det, from);
+    uint64_t seg = find_segment(s, t, p, det);
     if (seg == kFail) {
       if (veb::looks_empty(s->seg_tree)) {
         // nothing free: poll without taking the class lock, whose release
@@ -500,7 +498,7 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
       }
     }
     uint64_t got = kNullBlock;
-    if (wave::is_first()) got = get_block(s, seg, t, p);
+    if (wave::is_first()) got = get_block(s, seg, t, p, spins > 0);
     got = wave::readfirst(got);
     if (got >> 32) {
       // segment fully handed out: detach it and prefetch the next one
@@ -517,12 +515,6 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
     }
     const uint32_t blk = (uint32_t)got;
     if (blk != kNullBlock) return blk;
-    // the segment had no block for us (or a concurrent claim won): let the
-    // others finish and look at the next segment rather than this one again
-    if (!det) {
-      utils::backoff();
-      from = seg + 1;
-    }
   }
   return kNullBlock;
 }
