@@ -347,6 +347,7 @@ __device__ __noinline__ void free_segments(gal_state* s, uint64_t seg, uint64_t 
   clear_chunk_id(s, seg);
   utils::fence_release();
   veb::return_multiple(s->seg_tree, seg, n);
+  add_stat(s, kStatSegmentsReleased, n);
 }
 
 // shared_block_storage.cuh:119-134 get_valid_block_index + get_my_block for
@@ -473,6 +474,7 @@ __device__ inline uint64_t find_segment(gal_state* s, uint32_t t, uint32_t p, bo
 __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t p,
                                              bool det, uint32_t polls) {
   int attempts = 0;
+  uint64_t released = 0;  // segments given back so far, at the last check
   const uint32_t lk = tree_lock(s, t, p);
   for (uint32_t spins = 0; attempts < (int)polls && spins < kSpinCap;
        ++spins) {
@@ -480,7 +482,16 @@ __device__ inline uint32_t request_new_block(gal_state* s, uint32_t t, uint32_t 
     if (seg == kFail) {
       if (veb::looks_empty(s->seg_tree)) {
         // nothing free: poll without taking the class lock, whose release
-        // writes back the L2 and which every waiting wave would queue on
+        // writes back the L2 and which every waiting wave would queue on.
+        // Every 128 polls, give up when no segment came back in between:
+        // then no free is in flight to wait for (blocks coming back to a
+        // class tree are seen by the search itself).
+        if ((attempts & 127) == 0) {
+          const uint64_t rel =
+              wave::readfirst((uint64_t)atomic_load(&s->stats[kStatSegmentsReleased]));
+          if (attempts > 0 && rel == released) break;
+          released = rel;
+        }
         ++attempts;
         utils::backoff();
         continue;
