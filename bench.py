@@ -295,6 +295,22 @@ def zipf_sweep(args, G, torch):
     # fences (GAL_FLAG_NO_PAYLOAD_FENCE)
     best, table = autotune.tune(stream, args.zipf_pool_gib << 30, seed=args.seed,
                                 reps=args.zipf_reps, flags=G.GAL_FLAG_NO_PAYLOAD_FENCE)
+    launches = None
+    if best is not None:
+        # the same stream with one launch per batch (plain per-thread and the
+        # best batched form), at the best point's geometry
+        alloc = G.Gallatin(best["segment_bytes"], best["smallest"], best["biggest"]) \
+            .generate_on_device(args.zipf_pool_gib << 30, args.seed, print_info=False,
+                                flags=G.GAL_FLAG_NO_PAYLOAD_FENCE)
+        try:
+            launches = {
+                "plain": autotune.measure_launches(alloc, stream, False, 256, args.zipf_reps),
+                "batched": autotune.measure_launches(alloc, stream,
+                                                     autotune.FORMS[best["form"]],
+                                                     best["threads"], args.zipf_reps),
+            }
+        finally:
+            alloc.free_on_device()
     del stream
     torch.cuda.empty_cache()
     if best is None:
@@ -327,6 +343,13 @@ def zipf_sweep(args, G, torch):
         "plain_same_geometry": plain[0]["pairs_per_s"] if plain else None,
         "plain_best": plain_best["pairs_per_s"],
         "batched_over_plain": best["pairs_per_s"] / plain_best["pairs_per_s"],
+        # one kernel launch per batch and one for the cleanup, the way a
+        # caller that launches a kernel per batch runs the stream
+        "per_batch_launch": None if launches is None else {
+            form: {k: r[k] for k in ("pairs_per_s", "ms_per_step", "misses", "violations")}
+            for form, r in launches.items()},
+        "persistent_batched_over_per_batch_launch_plain":
+            None if launches is None else best["pairs_per_s"] / launches["plain"]["pairs_per_s"],
         "outside_grid": None if best_outside is None else dict(
             {k: best_outside[k] for k in keys}, value=best_outside["pairs_per_s"],
             ms_per_step=best_outside["ms_per_step"]),
@@ -595,6 +618,9 @@ def compact(out):
                   "violations", "wave_time", "plain_best", "batched_over_plain",
                   "outside_grid", "error"),
             atomic_frac=bz.get("roofline", {}).get("atomic_frac"),
+            per_batch_launch_plain=((bz.get("per_batch_launch") or {}).get("plain")
+                                    or {}).get("pairs_per_s"),
+            over_per_batch_launch_plain=bz.get("persistent_batched_over_per_batch_launch_plain"),
             cpu_glibc=(out.get("cpu_glibc_batched_zipf") or {}).get("value"))
     sv = out.get("serving")
     if sv:

@@ -24,7 +24,7 @@ import numpy as np
 
 from . import (CTR_BAD, CTR_COUNT, CTR_FREE_TICKS, CTR_FREES_SKIPPED, CTR_MALLOC_TICKS,
                CTR_MALLOCS, CTR_MISSES, CTR_PAIRS, CTR_TIMEOUTS, CTR_WAIT_POLLS, CTR_WAIT_TICKS,
-               CTR_WAVE_TICKS, BatchPlan)
+               CTR_WAVE_TICKS, BatchPlan, run_batches)
 from . import Gallatin, GallatinError
 
 SEGMENTS = (4 << 20, 16 << 20, 64 << 20)
@@ -96,6 +96,41 @@ def measure(alloc, stream, batched, threads, reps=3, warmup=1, split_reps=3):
         "events": {k: s1[k] - s0[k] for k in s1 if s1[k] != s0[k]},
         "wave_time": split,
         "wait_polls": c[CTR_WAIT_POLLS] / reps,
+    }
+
+
+def measure_launches(alloc, stream, batched, threads, reps=3, warmup=1):
+    """Time `reps` runs of `stream` with one launch per batch and one for
+    the cleanup (gal_run_batches), the way a caller that launches a kernel
+    per batch drives the allocator.  Same result keys as measure(), without
+    the event counters and the wave split."""
+    import torch
+    ctr = torch.zeros(CTR_COUNT, dtype=torch.int64, device="cuda")
+    for _ in range(warmup):
+        run_batches(alloc, stream, ctr, threads=threads, batched=batched)
+    torch.cuda.synchronize()
+    ctr.zero_()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    evs[0].record()
+    for k in range(reps):
+        run_batches(alloc, stream, ctr, threads=threads, batched=batched)
+        evs[k + 1].record()
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    total_ms = evs[0].elapsed_time(evs[-1])
+    steps = sorted(evs[k].elapsed_time(evs[k + 1]) for k in range(reps))
+    ok = (c[CTR_MALLOCS] == reps * stream.n_mallocs and c[CTR_BAD] == 0
+          and c[CTR_TIMEOUTS] == 0
+          and c[CTR_PAIRS] == c[CTR_MALLOCS] - c[CTR_MISSES]
+          and (c[CTR_MISSES] > 0 or c[CTR_FREES_SKIPPED] == 0))
+    return {
+        "pairs_per_s": c[CTR_PAIRS] / (total_ms / 1e3),
+        "ms_per_step": total_ms / reps,
+        "p50_step_ms": steps[len(steps) // 2],
+        "p99_step_ms": steps[-1],
+        "misses": c[CTR_MISSES],
+        "violations": c[CTR_BAD] + c[CTR_TIMEOUTS] + (0 if ok else 1),
     }
 
 

@@ -62,6 +62,10 @@ def _stub_result():
                                fresh_after_drain=True, steps=20),
         "batched_zipf": dict(sec, best=best, plain_best=2.6e9, batched_over_plain=1.1,
                              outside_grid=dict(best, value=3.0e9, ms_per_step=0.75),
+                             per_batch_launch={f: dict(pairs_per_s=4.0e8, ms_per_step=5.7,
+                                                       misses=0, violations=0)
+                                               for f in ("plain", "batched")},
+                             persistent_batched_over_per_batch_launch_plain=7.7,
                              table=[dict(row) for _ in range(60)], grid="g" * 100),
         "serving": {"p50_step_ms": 2.1, "p99_step_ms": 2.5, "mean_step_ms": 2.2,
                     "table_ops_per_s": 9.7e8, "words_per_s": 1.4e10,
