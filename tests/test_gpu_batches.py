@@ -88,6 +88,22 @@ def test_autotune_sweeps_both_batched_forms(gpu, small_stream):
     assert best is not None and best["path"] == "batched"
 
 
+def test_per_batch_launch_measure(gpu, small_stream):
+    """The bench's per-batch-launch timing (one launch per batch and one for
+    the cleanup) counts every pair of the stream, for the per-thread and the
+    workgroup form, on one instance; the instance drains to the fresh
+    state."""
+    from gallatin_amd import autotune
+    a = gpu.Gallatin(16 << 20, 16, 4096).generate_on_device(8 << 30, 9, print_info=False)
+    for batched, threads in ((False, 256), (1, 1024)):
+        r = autotune.measure_launches(a, small_stream, batched, threads, reps=2, warmup=1)
+        assert r["violations"] == 0 and r["misses"] == 0, r
+        assert r["pairs_per_s"] > 0 and r["p50_step_ms"] <= r["p99_step_ms"]
+    a.drain()
+    H.check_fresh(a)
+    a.free_on_device()
+
+
 @pytest.mark.parametrize("batched", [True, False])
 def test_timed_plan_splits_wave_time(gpu, small_stream, batched):
     """A plan made with timed=True (GAL_BATCH_TIMED) serves the stream the
