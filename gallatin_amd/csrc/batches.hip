@@ -289,12 +289,14 @@ __global__ __launch_bounds__(256) void k_table_free_plain(gal_state* s, uint64_t
 // never span two batches, so a free never waits on a malloc of its own
 // chunk.  Workgroups take chunks in stream order from a queue counter and
 // run them at once: a free waits only for its own table entry.  A chunk is
-// only taken by a running workgroup and a free's malloc lies in a chunk
-// taken earlier, so the earliest unfinished chunk never waits and the grid
-// always drains; the waits are bounded all the same (counter [5]).  A
-// chunk serves and publishes all its mallocs before any of its frees
-// waits, which shortens the chain of waits through the stream; the batched
-// forms batch the mallocs across the workgroup and free per wave.
+// only taken by a running workgroup, a free's malloc lies in a chunk taken
+// earlier, and a workgroup serves a chunk's mallocs as soon as it takes the
+// chunk, without waiting on any entry, so every entry is published and the
+// grid always drains; the waits are bounded all the same (counter [5]).  A
+// workgroup serves and publishes a chunk's mallocs before it runs the
+// frees of its previous chunk, which shortens the chain of waits through
+// the stream; the batched forms batch the mallocs across the workgroup and
+// free per wave.
 struct stream_chunk {
   uint64_t begin;  // first op
   uint32_t count;  // ops
@@ -318,22 +320,28 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
   if (kMode == 1) batch::batch_init(lds);
   if (kMode == 2) cache::cache_init(sc);
   op_counts c;
+  // A chunk's frees run after the next chunk's mallocs: they wait for
+  // entries that other chunks publish, and meanwhile this workgroup serves
+  // and publishes the entries that later chunks' frees wait for.
+  stream_chunk prev{0, 0, 0};  // count 0: no frees pending
   for (;;) {
     if (threadIdx.x == 0) cur = utils::atomic_add(queue, 1u);
     __syncthreads();
     const uint32_t k = cur;
     __syncthreads();  // every thread has read cur before it is written again
-    if (k >= nchunks) break;
-    const stream_chunk ch = chunks[k];
+    const bool done = k >= nchunks;
+    const stream_chunk ch = done ? stream_chunk{0, 0, 0} : chunks[k];
     const uint32_t rounds = (ch.count + kThreads - 1) / kThreads;
-    // Every malloc of the chunk is served and published before any of its
-    // frees waits: the frees of later chunks wait on exactly these entries,
-    // and a chunk's frees only name entries of earlier batches.  Serving
-    // them round by round instead (malloc, then wait and free, per round)
-    // held a chunk's later mallocs behind its own earlier waits, so waits
-    // chained from chunk to chunk through the stream.
+    // Every malloc of the chunk is served and published before any free of
+    // the workgroup waits: the frees of later chunks wait on exactly these
+    // entries, and a chunk's frees only name entries of earlier batches.
+    // Serving them round by round instead (malloc, then wait and free, per
+    // round) held a chunk's later mallocs behind its own earlier waits, so
+    // waits chained from chunk to chunk through the stream.
     const uint64_t t0 = timed ? wall_clock64() : 0;
-    if constexpr (kMode == 1) {
+    if (done) {
+      // nothing to serve: only the last chunk's frees are left
+    } else if constexpr (kMode == 1) {
       // The workgroup form counts the requests of all the chunk's rounds
       // first and claims them together: two barriers and one claim per
       // class for the chunk, where one batch_malloc per round took two
@@ -395,12 +403,13 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
     // whose entry is not yet published, to wait for them after the chunk's
     // other frees, moved wave time from waiting to freeing and did not
     // shorten a replay.)
-    for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t prev_rounds = (prev.count + kThreads - 1) / kThreads;
+    for (uint32_t r = 0; r < prev_rounds; ++r) {
       const uint32_t j = r * kThreads + threadIdx.x;
       bool is_free = false;
       uint32_t slot = 0;
-      if (j < ch.count) {
-        const gal_batch_op op = ops[ch.begin + j];
+      if (j < prev.count) {
+        const gal_batch_op op = ops[prev.begin + j];
         is_free = op.kind == 1 && op.slot < nslots;  // invalid ops were counted above
         slot = op.slot;
       }
@@ -413,6 +422,8 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
         c.free_ticks += wall_clock64() - t2;
       }
     }
+    if (done) break;
+    prev = ch;
   }
   if (kMode == 2) cache::cache_flush(s, sc);
   flush_counts(ctr, c, timed);
