@@ -256,16 +256,32 @@ __device__ inline void take_left(uint64_t* ptrs, uint64_t i, bool in, op_counts&
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) void k_table_free_batched(gal_state* s, uint64_t* ptrs,
                                                                  uint64_t nslots, uint64_t* ctr) {
+  // Blocks the cleanup empties are returned together at the end (nothing
+  // allocates meanwhile): returning each on the spot held the workgroup at
+  // its next barrier for the one thread's ring return.
+  constexpr uint32_t kLater = 4 * kThreads;
   __shared__ batch::lds_state lds;
+  __shared__ uint32_t later[kLater];
+  __shared__ uint32_t nlater;
+  if (threadIdx.x == 0) nlater = 0;
   batch::batch_init(lds);
   op_counts c;
+  const uint32_t my_p = detail::wave_part(s);
   const uint64_t nchunks = (nslots + kThreads - 1) / kThreads;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const uint64_t i = chunk * kThreads + threadIdx.x;
     void* old;
     take_left(ptrs, i, i < nslots, c, old);
-    batch::batch_free(s, old, lds);
+    const bool grouped = batch::free_stage(s, old, lds, my_p);
+    __syncthreads();
+    batch::free_scan_later(s, lds, my_p, later, &nlater, kLater);
+    const bool rest = old != nullptr && !grouped;
+    if (__ballot(rest)) {
+      if (rest) detail::free_impl(s, old);
+    }
+    __syncthreads();
   }
+  batch::return_later(s, my_p, later, nlater, kLater);
   flush_counts(ctr, c);
 }
 

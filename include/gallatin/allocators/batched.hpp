@@ -272,6 +272,43 @@ __device__ __forceinline__ void free_scan(gal_state* s, lds_state& L, uint32_t m
   }
 }
 
+// free_scan for a workgroup that nothing allocates from while it frees
+// (a cleanup): a block its frees empty is not returned on the spot, where
+// the one thread returning it would hold every thread of the workgroup at
+// the next barrier; its id goes to `later` (`cap` entries, count in
+// *nlater, both in LDS) and the workgroup returns the listed blocks
+// together after its last barrier (return_later).  Past the capacity a
+// block is returned at once.  An emptied block is no longer pinned, so no
+// slice of it is handed out before it is returned.
+__device__ __forceinline__ void free_scan_later(gal_state* s, lds_state& L, uint32_t my_p,
+                                                uint32_t* later, uint32_t* nlater,
+                                                uint32_t cap) {
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kFreeSlots; i += blockDim.x) {
+    const uint32_t key = L.key[i];
+    if (!key) continue;
+    const uint32_t b = key - 1, k = L.cnt[i];
+    L.key[i] = 0;
+    L.cnt[i] = 0;
+    const uint32_t old = utils::atomic_add(&detail::block_at(s, b)->free_counter, k);
+    if (old + k == kSlicesPerBlock) {
+      utils::atomic_exch(&detail::block_at(s, b)->free_counter, 0u);
+      const uint32_t at = atomicAdd(nlater, 1u);
+      if (at < cap)
+        later[at] = b;
+      else
+        detail::free_block(s, b, my_p);
+    }
+  }
+}
+
+// The blocks free_scan_later listed, returned by the workgroup's threads
+// together; call after a barrier that follows the last free_scan_later.
+__device__ inline void return_later(gal_state* s, uint32_t my_p, const uint32_t* later,
+                                    uint32_t nlater, uint32_t cap) {
+  const uint32_t n = nlater < cap ? nlater : cap;
+  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) detail::free_block(s, later[j], my_p);
+}
+
 // ---- workgroup entry points ----
 
 // Each thread with `want` gets a pointer of `size` bytes (or nullptr).
