@@ -766,6 +766,11 @@ struct gal_serving {
   uint32_t* tokens = nullptr;           // device, 1024 entries
   uint64_t* free_segments = nullptr;    // device
   table_large* table = nullptr;
+  // the table pass runs beside the session pass on a stream of its own
+  // (default priority), forked from and joined back to the step's stream
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool sessions_ready = false;  // k_serve_init has run: sessions may own memory
 };
 
 namespace {
@@ -1105,12 +1110,19 @@ int gal_serving_create(uint32_t max_sessions, uint64_t seed, const uint32_t* tok
   }
   uint64_t m = ((uint64_t)max_sessions * 2654435769ULL >> 32) | 1;  // ~0.618 n
   while (std::gcd(m, (uint64_t)max_sessions) != 1) m += 2;
+  if ((e = hipStreamCreateWithFlags(&sv->side, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&sv->fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&sv->join, hipEventDisableTiming)) != hipSuccess) {
+    gal_serving_destroy(sv);
+    return hip_fail(e, "serving streams");
+  }
   k_serve_init<<<(max_sessions + 255) / 256, 256>>>(sv->sessions, sv->free_slots, sv->free_top,
                                                     max_sessions, m % max_sessions ? m : 1);
   if ((e = hipDeviceSynchronize()) != hipSuccess) {
     gal_serving_destroy(sv);
     return hip_fail(e, "serving init");
   }
+  sv->sessions_ready = true;
   sv->table = table_large::generate_on_device(seed);
   if (!sv->table) {
     gal_serving_destroy(sv);
@@ -1130,6 +1142,19 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  // The table operations depend on nothing of the session pass: they run on
+  // the side stream, after the earlier work of `st`, and `st` waits for
+  // them at the end of the step.
+  hipError_t e = hipSuccess;
+  if (ht_ops) {
+    if ((e = hipEventRecord(sv->fork, st)) != hipSuccess ||
+        (e = hipStreamWaitEvent(sv->side, sv->fork, 0)) != hipSuccess)
+      return hip_fail(e, "serving fork");
+    k_serve_table<<<(ht_ops + 255) / 256, 256, 0, sv->side>>>(sv->table, sv->seed, step, ht_ops,
+                                                              counters_dev);
+    if ((e = hipEventRecord(sv->join, sv->side)) != hipSuccess)
+      return hip_fail(e, "serving join");
+  }
   k_serve_reset<<<1, 64, 0, st>>>(sv->queue, sv->free_segments);
   const uint32_t waves = (sv->max_sessions + 63) / 64;
   uint32_t groups = (uint32_t)cus * 8;
@@ -1143,10 +1168,9 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
                                                           sv->free_top, arrivals, reserve_segments,
                                                           gate, sv->free_segments, step,
                                                           counters_dev);
-  if (ht_ops)
-    k_serve_table<<<(ht_ops + 255) / 256, 256, 0, st>>>(sv->table, sv->seed, step, ht_ops,
-                                                        counters_dev);
-  hipError_t e = hipGetLastError();
+  if (ht_ops && (e = hipStreamWaitEvent(st, sv->join, 0)) != hipSuccess)
+    return hip_fail(e, "serving join");
+  e = hipGetLastError();
   return e == hipSuccess ? GAL_OK : hip_fail(e, "serving step");
 }
 
@@ -1195,7 +1219,8 @@ int gal_serving_table(gal_serving* sv, uint64_t* keys, uint64_t* buckets, uint64
 
 int gal_serving_destroy(gal_serving* sv) {
   if (!sv) return GAL_OK;
-  if (sv->sessions) {
+  if (sv->side) (void)hipStreamSynchronize(sv->side);
+  if (sv->sessions && sv->sessions_ready) {  // before set-up, no session owns memory
     k_serve_free_all<<<(sv->max_sessions + 255) / 256, 256>>>(sv->sessions, sv->max_sessions);
     (void)hipDeviceSynchronize();
   }
@@ -1206,6 +1231,9 @@ int gal_serving_destroy(gal_serving* sv) {
   if (sv->queue) (void)hipFree(sv->queue);
   if (sv->free_segments) (void)hipFree(sv->free_segments);
   if (sv->tokens) (void)hipFree(sv->tokens);
+  if (sv->fork) (void)hipEventDestroy(sv->fork);
+  if (sv->join) (void)hipEventDestroy(sv->join);
+  if (sv->side) (void)hipStreamDestroy(sv->side);
   delete sv;
   return GAL_OK;
 }

@@ -382,11 +382,13 @@ int gal_queue_dequeue(gal_queue *q, uint64_t *out_dev, uint8_t *ok_dev, uint64_t
  * Device-resident sessions, each owning a chunked_vector<uint64_t, 16,
  * 8192, 512> (include/gallatin/data_structs/chunked_vector.hpp: chunks of
  * 128 B doubling to 64 KiB, then up to 512 more 64 KiB chunks) and a shared
- * gal_table of kind 1.  A step (three launches on `stream`): each live
+ * gal_table of kind 1.  A step is enqueued on `stream`: each live
  * session ends with probability end_q32 / 2^32 (vector freed) or appends
  * tokens x 32 words, tokens drawn from tokens_table[1024]; then `arrivals`
  * new sessions are admitted if the allocator has more than
- * reserve_segments free segments (else refused); then ht_ops table
+ * reserve_segments free segments (else refused).  Alongside, on a second
+ * stream of the serving object that starts after `stream`'s earlier work
+ * and that `stream` waits for before the step ends, run ht_ops table
  * operations, 70 % insert / 30 % remove, keys uniform in [1, 2^28].
  * gate 0 counts free segments as the segment tree's set bits; gate 1 counts
  * the free memory in segments: those free segments plus the free blocks of
