@@ -1144,17 +1144,11 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // The table operations depend on nothing of the session pass: they run on
   // the side stream, after the earlier work of `st`, and `st` waits for
-  // them at the end of the step.
+  // them at the end of the step.  The session pass is submitted first, so
+  // that its workgroups are dispatched ahead of the table's.
   hipError_t e = hipSuccess;
-  if (ht_ops) {
-    if ((e = hipEventRecord(sv->fork, st)) != hipSuccess ||
-        (e = hipStreamWaitEvent(sv->side, sv->fork, 0)) != hipSuccess)
-      return hip_fail(e, "serving fork");
-    k_serve_table<<<(ht_ops + 255) / 256, 256, 0, sv->side>>>(sv->table, sv->seed, step, ht_ops,
-                                                              counters_dev);
-    if ((e = hipEventRecord(sv->join, sv->side)) != hipSuccess)
-      return hip_fail(e, "serving join");
-  }
+  if (ht_ops && (e = hipEventRecord(sv->fork, st)) != hipSuccess)
+    return hip_fail(e, "serving fork");
   k_serve_reset<<<1, 64, 0, st>>>(sv->queue, sv->free_segments);
   const uint32_t waves = (sv->max_sessions + 63) / 64;
   uint32_t groups = (uint32_t)cus * 8;
@@ -1162,6 +1156,14 @@ int gal_serving_step(gal_serving* sv, uint32_t step, uint32_t arrivals, uint64_t
   k_serve_sessions<<<groups, 256, 0, st>>>(sv->sessions, sv->max_sessions, sv->free_slots,
                                            sv->free_top, sv->queue, sv->tokens, sv->seed, step,
                                            end_q32, counters_dev);
+  if (ht_ops) {
+    if ((e = hipStreamWaitEvent(sv->side, sv->fork, 0)) != hipSuccess)
+      return hip_fail(e, "serving fork");
+    k_serve_table<<<(ht_ops + 255) / 256, 256, 0, sv->side>>>(sv->table, sv->seed, step, ht_ops,
+                                                              counters_dev);
+    if ((e = hipEventRecord(sv->join, sv->side)) != hipSuccess)
+      return hip_fail(e, "serving join");
+  }
   k_serve_free_segments<<<1, 1024, 0, st>>>(sv->free_segments);
   if (arrivals)
     k_serve_admit<<<(arrivals + 255) / 256, 256, 0, st>>>(sv->sessions, sv->free_slots,

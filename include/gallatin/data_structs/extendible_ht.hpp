@@ -215,6 +215,13 @@ struct extendible_hash_table {
   static __device__ __forceinline__ void st(uint64_t* p, uint64_t v) {
     gallatin::utils::atomic_store(p, v);
   }
+  // A read that may return an old value: a workgroup-scope load, served by
+  // this XCD's caches.  (A volatile read compiles to a system-scope load,
+  // which goes to memory like a device-coherent one.)
+  static __device__ __forceinline__ uint64_t ld_cached(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 
   __device__ uint64_t hash(Key k) const { return hashers::MurmurHash64A_u64((uint64_t)k, seed); }
 
@@ -248,7 +255,7 @@ struct extendible_hash_table {
   // from the whole device queue at one memory channel.
   static constexpr int kOwnerFan = 4;
   __device__ bucket* owner(uint64_t h, bool fresh) {
-    uint64_t top = fresh ? ld(&level) : *reinterpret_cast<volatile uint64_t*>(&level);
+    uint64_t top = fresh ? ld(&level) : ld_cached(&level);
     for (;;) {
       uint64_t e[kOwnerFan];
 #pragma unroll
@@ -310,7 +317,7 @@ struct extendible_hash_table {
     __builtin_amdgcn_s_waitcnt(0);  // the new bucket is whole before it is published
     st(reinterpret_cast<uint64_t*>(&directory[home]), reinterpret_cast<uint64_t>(nb));
     // level only grows: a cached value at or above d + 1 needs no update
-    if (*reinterpret_cast<volatile uint64_t*>(&level) < d + 1)
+    if (ld_cached(&level) < d + 1)
       gallatin::utils::atomic_max(&level, (uint64_t)(d + 1));
     __builtin_amdgcn_s_waitcnt(0);
     st(&b->ctl, ((uint64_t)(d + 1) << 32) | 1);  // still locked
