@@ -184,7 +184,8 @@ def tune(stream, pool_bytes, seed=42, reps=3, warmup=1, segments=SEGMENTS,
 
 def main(argv=None):
     """python -m gallatin_amd.autotune [--seed S] [--pool-gib G] [--reps R]
-    [--segments-mib M ...] [--smallest B ...] [--threads T ...] [--forms F ...]:
+    [--segments-mib M ...] [--smallest B ...] [--threads T ...] [--forms F ...]
+    [--slots-min K]:
     generate the Zipf stream, sweep the given points (the full grid by
     default), print one JSON line per point and the best point last."""
     import json
@@ -200,7 +201,7 @@ def main(argv=None):
     stream = BatchStream(ops, off, nslots)
     best, _ = tune(stream, a.pool_gib << 30, seed=a.seed, reps=a.reps, flags=a.flags,
                    segments=tuple(m << 20 for m in a.segments_mib), smallest=tuple(a.smallest),
-                   threads=tuple(a.threads), forms=tuple(a.forms),
+                   threads=tuple(a.threads), forms=tuple(a.forms), slots_min=a.slots_min,
                    log=lambda r: print(json.dumps(r), flush=True))
     print(json.dumps({"best": best}), flush=True)
 
@@ -219,6 +220,8 @@ def parse_args(argv=None):
     p.add_argument("--smallest", type=int, nargs="+", default=list(SMALLEST))
     p.add_argument("--threads", type=int, nargs="+", default=list(THREADS))
     p.add_argument("--forms", nargs="+", choices=tuple(FORMS), default=list(FORMS))
+    p.add_argument("--slots-min", type=int, default=32,
+                   help="pinned blocks per class at least (gal_config.slots_min)")
     return p.parse_args(argv)
 
 
