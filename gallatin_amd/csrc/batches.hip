@@ -395,7 +395,6 @@ __global__ __launch_bounds__(kThreads) void k_stream(gal_state* s, const gal_bat
           if (rest) w = detail::malloc_impl(s, size);
           if (rest) np = w;
         }
-        if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
         give_op(np, want, slot, ptrs, c);
       }
     } else {

@@ -327,8 +327,7 @@ __device__ inline void* batch_malloc(gal_state* s, bool want, uint64_t size, lds
     if (rest) q = detail::malloc_impl(s, size);
     if (rest) out = q;
   }
-  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
-  return out;
+  return out;  // no acquire: see malloc_impl
 }
 
 // Each thread frees `ptr` (nullptr: nothing).
@@ -371,8 +370,7 @@ __device__ inline void* batch_step(gal_state* s, void* old_ptr, bool want, uint6
     if (rest) q = detail::malloc_impl(s, size);
     if (rest) out = q;
   }
-  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1) utils::fence_acquire();
-  return out;
+  return out;  // no acquire: see malloc_impl
 }
 
 }  // namespace batch

@@ -23,7 +23,8 @@
 //    (agent-scope release).  A segment changes home only when a wave of its
 //    home XCD releases it, after writing its L2 back.
 //  * With one partition (deterministic mode, or partitions = 1) every free
-//    is an agent-scope release and every malloc an agent-scope acquire.
+//    is an agent-scope release.  A malloc needs no acquire in either case
+//    (malloc_impl says why).
 //
 // Device API (same names as the reference): Gallatin<S,m,M>::malloc / free /
 // get_allocated_size / owns_allocation; host API: generate_on_device /
@@ -838,12 +839,15 @@ __device__ __forceinline__ void* malloc_impl(gal_state* s, uint64_t size, bool w
     const uint64_t r = malloc_segments(s, path == kPathSegment, nseg, det);
     if (path == kPathSegment) bytes = r;
   }
-  // With one partition a slice may come from any XCD's past: acquire at
-  // agent scope.  With XCD partitions the previous users of a slice wrote
-  // through this XCD's L2 or were written back before their free.
-  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1 &&
-      __ballot(bytes != kFail))
-    utils::fence_acquire();
+  // No acquire on the way out.  The previous user of a slice released its
+  // writes before its free (free_impl writes its XCD's L2 back, or the
+  // slice stayed on its home XCD), so none of its bytes can land on the new
+  // user's data later.  This XCD's L2 may still hold clean old copies of
+  // the slice's lines; the L2 writes back only the bytes written, so those
+  // copies show only in bytes the new user reads before writing them, which
+  // malloc leaves undefined.  The allocator's own words are all accessed
+  // with device-coherent atomics.  (An agent-scope acquire here invalidated
+  // the XCD's L2 on every malloc of a one-partition instance.)
   return bytes == kFail ? nullptr : (void*)(s->memory + bytes);
 }
 

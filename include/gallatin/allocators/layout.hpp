@@ -50,7 +50,7 @@ constexpr uint32_t kSpinCap = 1u << 16;
 
 // flags
 constexpr uint32_t kFlagDeterministic = 1u;  // smallest-id segment, slot 0
-constexpr uint32_t kFlagNoPayloadFence = 2u; // skip release/acquire on payload
+constexpr uint32_t kFlagNoPayloadFence = 2u; // skip the payload release at free
 
 // seg_state packing: bits 0-15 tree id (kFreeTree = unformatted), bits
 // 16-23 home partition, bits 32-63 active count (int32)

@@ -159,10 +159,7 @@ __device__ inline void* cache_malloc(gal_state* s, slice_cache& L, bool want, ui
       if ((int)lane == l) atomicExch(&L.w[c], nw);
     }
   }
-  const bool got = off != kFail;
-  // With one partition a slice may come from any XCD's past (as malloc_impl)
-  if (!(s->flags & kFlagNoPayloadFence) && s->num_parts == 1 && __ballot(got))
-    utils::fence_acquire();
+  const bool got = off != kFail;  // no acquire: see malloc_impl
   void* res = got ? (void*)(s->memory + detail::offset_to_bytes(s, off, t)) : nullptr;
   if (want && !got) res = detail::malloc_impl(s, size);
   return res;
