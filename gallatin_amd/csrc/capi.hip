@@ -333,7 +333,7 @@ int gal_create(const gal_config* cfg, gal_allocator** out) {
   const uint64_t o_blocks = take((1ULL << h.block_bits) * sizeof(gal_block));
   const uint64_t o_slots = take((uint64_t)total * sizeof(uint32_t));
   const uint64_t o_locks = take((1 + (uint64_t)T * P) * sizeof(uint32_t));
-  const uint64_t o_stats = take(kStatCount * sizeof(unsigned long long));
+  const uint64_t o_stats = take(kStatWords * sizeof(unsigned long long));
   const uint64_t o_trees = take(words * sizeof(uint64_t));
   a->meta_bytes = off;
   hipError_t e = hipMalloc(&a->meta, off);

@@ -719,7 +719,32 @@ __device__ inline bool mark_multi_head(gal_state* s, uint64_t idx, uint64_t n) {
   }
   add_error(s, kStatErrMultiHead);
   veb::return_multiple(s->seg_tree, idx, n);
+  add_stat(s, kStatSegmentsReleased, n);  // segments back in the tree (see no_run_known)
   return false;
+}
+
+// A search for a run of n free segments that found none stays true until a
+// segment goes back into the segment tree, and every path that puts one
+// back adds to stat [3] after doing so.  So the smallest length a search
+// found no run for, with stat [3] as read before that search, lets later
+// requests of that length or more fail without taking the lock and scanning
+// the tree again while stat [3] is unchanged.  (Between a return and its
+// count a request can fail that would have found the returned segment; a
+// search racing that return can miss it as well.)  Not used in the
+// deterministic mode, whose replay scans as the reference does.
+// Both are wave-uniform: every lane of the wave takes part in the search.
+__device__ inline uint64_t released_count(const gal_state* s) {
+  return wave::readfirst((uint64_t)atomic_load(&s->stats[kStatSegmentsReleased]));
+}
+__device__ inline bool no_run_known(const gal_state* s, uint64_t rel, uint64_t n) {
+  const uint64_t h = wave::readfirst((uint64_t)atomic_load(&s->stats[kStatSegHint]));
+  return h != 0 && (h >> 16) == (rel & 0xFFFFFFFFFFFFULL) && n >= (h & 0xFFFF);
+}
+__device__ inline void note_no_run(gal_state* s, uint64_t rel, uint64_t n) {
+  const uint64_t mine = ((rel & 0xFFFFFFFFFFFFULL) << 16) | (n & 0xFFFF);
+  const uint64_t h = atomic_load(&s->stats[kStatSegHint]);
+  if ((h >> 16) != (mine >> 16) || (h & 0xFFFF) > n)
+    atomic_store(&s->stats[kStatSegHint], (unsigned long long)mine);
 }
 
 // Take the multi-segment lock (lock 0, gallatin.cuh:796).
@@ -737,11 +762,15 @@ __device__ inline bool lock_segments(gal_state* s) {
 __device__ __noinline__ uint64_t malloc_segment_path(gal_state* s, uint64_t n) {
   const uint32_t T = s->num_trees;
   if (n == 0 || n > s->num_segments || n + T + 1 >= kFreeTree) return kFail;
+  const bool det = deterministic(s);
   for (uint32_t attempt = 0; attempt < s->lim_segment; ++attempt) {
+    const uint64_t rel = released_count(s);
+    if (!det && no_run_known(s, rel, n)) return kFail;
     if (!lock_segments(s)) return kFail;
     const uint64_t idx = veb::gather_multiple(s->seg_tree, n);
     if (idx == kFail) {  // no free run of n segments now
       unlock(s, 0);
+      if (!det && wave::is_first()) note_no_run(s, rel, n);
       return kFail;
     }
     uint32_t ok = 0;
@@ -771,13 +800,17 @@ __device__ __noinline__ uint64_t malloc_segments(gal_state* s, bool want, uint64
   if (!det && wave::count(wm) > 1 && !__ballot(want && !ok_n)) {
     uint32_t pre = 0, total = 0;
     wave::group_scan(want ? (uint32_t)n : 0u, wm, pre, total);
-    if (total <= s->num_segments && lock_segments(s)) {
+    const uint64_t rel = released_count(s);
+    const bool hinted = total <= 0xFFFF;  // the hint holds 16-bit lengths
+    if (total <= s->num_segments && !(hinted && no_run_known(s, rel, total)) &&
+        lock_segments(s)) {
       const uint64_t idx = veb::gather_multiple(s->seg_tree, total);
       unlock(s, 0);
       if (idx != kFail) {
         if (want && mark_multi_head(s, idx + pre, n)) out = (idx + pre) << s->seg_shift;
         return out;
       }
+      if (hinted && wave::is_first()) note_no_run(s, rel, total);
     }
   }
   uint64_t no_run = ~0ULL;  // smallest size that found no run

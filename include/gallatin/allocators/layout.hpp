@@ -129,7 +129,11 @@ enum gal_stat {
   kStatMissRetries = 15,    // slice requests failed: retries used up
   kStatRingRuns = 16,       // ring completions that covered several tickets
   kStatRingRunTickets = 17, // tickets those completions covered
-  kStatCount = 18
+  kStatCount = 18,
+  // internal, after the reported counters: the last multi-segment search
+  // that found no run, as (stat [3] when it started) << 16 | run length
+  kStatSegHint = 18,
+  kStatWords = 19
 };
 
 // The instance viewed through the reference's alloc_table accessors
@@ -168,7 +172,7 @@ struct gal_state {
   uint32_t* slots;               // pinned blocks of all size classes
   uint32_t* locks;               // [1 + T * parts]: multi-segment lock, then one
                                  // per class tree (gallatin.cuh:314, 1513-1519)
-  unsigned long long* stats;     // [kStatCount]
+  unsigned long long* stats;     // [kStatWords]: kStatCount reported, then internal words
   uint32_t slot_off[kMaxTrees + 1];
   uint32_t slot_count[kMaxTrees];  // multiple of num_parts; partition p owns
                                    // slots [p, p+1) * count / num_parts
