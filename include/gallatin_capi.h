@@ -129,8 +129,9 @@ int gal_get_info(gal_allocator *a, gal_info *out);
  * reference prints errors instead).  Copies min(n, GAL_STAT_COUNT) values
  * and returns GAL_STAT_COUNT.  [0] errors (sum of [6..13]), [1] boot
  * failed, [2] segments formatted, [3] segments given back to the free
- * pool (single segments whose last block came back, and the segments of
- * multi-segment frees), [4] blocks
+ * pool (single segments whose last block came back, the segments of
+ * multi-segment frees, and runs a failed multi-segment claim gave back),
+ * [4] blocks
  * returned, [5] empty pinned slots refilled, [6] chunk id reset
  * failed, [7] ring completion timed out, [8] block freed into an
  * unformatted segment, [9] empty ring entry, [10] segment set-up failed,
