@@ -65,6 +65,8 @@ def parse():
                    help="skip the block and multi-segment batches")
     p.add_argument("--big-steps", type=int, default=20)
     p.add_argument("--no-serving", action="store_true", help="skip the serving loop")
+    p.add_argument("--no-word-churn", action="store_true",
+                   help="skip the churn with one word per allocation")
     p.add_argument("--serving-pool-gib", type=int, default=200)
     # the timed window covers a whole admission cycle: the pool fills, the
     # gate closes, long-lived sessions drain, the gate reopens
@@ -508,6 +510,46 @@ def run_path(args, G, torch, path):
     }
 
 
+def churn_one_word(args, G, torch):
+    """The headline's request stream with one 64-bit word of each allocation
+    written and checked, the pattern of the reference's churn test, through
+    the workgroup-batched path and the per-thread path: the allocator rather
+    than HBM sets the rate."""
+    res = {"workload": f"mixed-size churn: {args.pool_gib} GiB pool, {args.threads} threads "
+                       f"x {args.rounds} rounds, log-uniform 1..4095 B, 50% replace per round, "
+                       "one u64 written and checked per allocation"}
+    for name, batched, threads in (("batched", True, args.batch_threads),
+                                   ("plain", False, args.block_threads)):
+        alloc = G.Gallatin(16 << 20, 16, 4096).generate_on_device(
+            args.pool_gib << 30, args.seed, print_info=False, flags=args.flags,
+            slots0=args.slots0, slots_min=args.slots_min, partitions=args.partitions)
+        ctr = torch.zeros(G.CTR_COUNT, dtype=torch.int64, device="cuda")
+        for w in range(args.warmup):
+            alloc.run_churn_word(args.seed + w, args.threads, args.rounds, counters=ctr,
+                                 batched=batched, threads=threads)
+        torch.cuda.synchronize()
+        ctr.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for k in range(args.steps):
+            alloc.run_churn_word(args.seed + 100 + k, args.threads, args.rounds, counters=ctr,
+                                 batched=batched, threads=threads)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        c = ctr.cpu().tolist()
+        info = alloc.info()
+        alloc.free_on_device()
+        res[name] = {"value": c[G.CTR_PAIRS] / (ms / 1e3), "ms_per_step": ms / args.steps,
+                     "misses": c[G.CTR_MISSES], "violations": c[G.CTR_BAD] + info["errors"],
+                     "kernel": (f"k_churn_batched_word<{threads}>" if batched
+                                else "k_churn_word")}
+    res["value"] = res["batched"]["value"]
+    res["batched_over_plain"] = res["batched"]["value"] / res["plain"]["value"]
+    return res
+
+
 def section_roofline(pairs_per_s, bytes_per_pair, atomic_peak):
     """Atomic and HBM roofline fractions of a section: two algorithmic
     atomics per pair (one claim, one release) against the measured
@@ -526,7 +568,7 @@ def add_section_rooflines(out, atomic_peak):
         for sec in (ps, ps.get("coherent_payload"), ps.get("partitioned_lazy_boot")):
             if sec:
                 sec["roofline"] = section_roofline(sec["value"], 8, atomic_peak)
-    for key, per_pair in (("block_multiseg", 16), ("batched_zipf", 16)):
+    for key, per_pair in (("block_multiseg", 16), ("batched_zipf", 16), ("churn_one_word", 16)):
         sec = out.get(key)
         if sec and "value" in sec:
             sec["roofline"] = section_roofline(sec["value"], per_pair, atomic_peak)
@@ -605,6 +647,14 @@ def compact(out):
             coherent_payload=(ps.get("coherent_payload") or {}).get("value"),
             partitioned_lazy_boot=(ps.get("partitioned_lazy_boot") or {}).get("value"),
             cpu_glibc=(out.get("cpu_glibc_plain_small") or {}).get("value"))
+    cw = out.get("churn_one_word")
+    if cw:
+        w["churn_one_word"] = dict(
+            value=cw["value"], plain=cw["plain"]["value"],
+            batched_over_plain=cw["batched_over_plain"],
+            violations=cw["batched"]["violations"] + cw["plain"]["violations"],
+            misses=cw["batched"]["misses"] + cw["plain"]["misses"],
+            atomic_frac=cw.get("roofline", {}).get("atomic_frac"))
     bm = out.get("block_multiseg")
     if bm:
         w["block_multiseg"] = dict(
@@ -643,8 +693,8 @@ def emit(out, detail_path=""):
     if detail_path:
         with open(detail_path, "w") as f:
             json.dump(out, f)
-    for key in ("plain_small", "block_multiseg", "serving", "hbm_ceilings", "tree_search",
-                "atomic_roofline"):
+    for key in ("plain_small", "churn_one_word", "block_multiseg", "serving", "hbm_ceilings",
+                "tree_search", "atomic_roofline"):
         if key in out:
             print(json.dumps({"section": key, **out[key]}), flush=True)
     bz = out.get("batched_zipf")
@@ -757,6 +807,9 @@ def main():
     out["plain_small"]["partitioned_lazy_boot"] = plain_small(
         args, G, torch, flags=G.GAL_FLAG_LAZY_BOOT, partitions=xcds)
     log("plain small done")
+    if not args.no_word_churn:
+        out["churn_one_word"] = churn_one_word(args, G, torch)
+        log("one-word churn done")
     if not args.no_big:
         out["block_multiseg"] = block_multiseg(args, G, torch)
         log("block and multi-segment done")

@@ -47,7 +47,8 @@ CAPI_SYMBOLS = [
     "gal_free_global", "gal_print_global_stats", "gal_global_handle",
     "gal_malloc_bulk", "gal_free_bulk", "gal_global_malloc_bulk",
     "gal_global_free_bulk", "gal_run_wave_stream", "gal_run_plain",
-    "gal_run_churn", "gal_run_churn_batched", "gal_run_big", "gal_bench_atomic",
+    "gal_run_churn", "gal_run_churn_batched", "gal_run_churn_word", "gal_run_big",
+    "gal_bench_atomic",
     "gal_bench_copy",
     "gal_bench_scatter", "gal_bench_tree",
     "gal_last_error", "gal_get_stats", "gal_run_batches", "gal_batch_plan_create",
@@ -136,6 +137,7 @@ def lib():
         L.gal_run_plain.argtypes = [_p, _u64, _u64, _p, _u32, _p]
         L.gal_run_churn.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _p]
         L.gal_run_churn_batched.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _p]
+        L.gal_run_churn_word.argtypes = [_p, _u64, _u64, _u32, _p, _u32, _u32, _u32, _p]
         L.gal_run_big.argtypes = [_p, _u64, _u32, _u64, _u64, _p, _p, _p]
         L.gal_bench_atomic.argtypes = [_p, _u64, _u64, _u32, _u32, _p]
         L.gal_bench_copy.argtypes = [_p, _p, _u64, _p]
@@ -381,6 +383,16 @@ class Gallatin:
         ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
         _check(lib().gal_run_churn(self.handle, seed, nthreads, rounds,
                                    ctr.data_ptr(), block_threads, _stream()))
+        return ctr
+
+    def run_churn_word(self, seed, nthreads, rounds, counters=None, batched=False,
+                       workgroups=0, threads=0):
+        """gal_run_churn_word: the churn stream with one word of each
+        allocation written and checked (the reference's churn test pattern),
+        per thread or through workgroup batches."""
+        ctr = counters if counters is not None else _dev_u64(CTR_COUNT)
+        _check(lib().gal_run_churn_word(self.handle, seed, nthreads, rounds, ctr.data_ptr(),
+                                        1 if batched else 0, workgroups, threads, _stream()))
         return ctr
 
     def run_big(self, seed, step, n_small=4096, n_large=64, ptrs=None, counters=None):

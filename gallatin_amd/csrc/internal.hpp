@@ -25,6 +25,15 @@ struct gal_allocator {
 namespace gallatin_capi {
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+
+// Workgroups of `threads` for n threads, and the status of the last launch.
+inline unsigned grid_for(uint64_t n, uint32_t threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GAL_OK : hip_fail(e, what);
+}
 }  // namespace gallatin_capi
 
 #endif

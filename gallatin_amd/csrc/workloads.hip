@@ -20,38 +20,7 @@ using namespace gallatin_capi;
 
 namespace {
 
-// Sum of a per-lane value over the wave (every lane of the wave active).
-__device__ inline uint64_t wave_sum(uint64_t v) {
-  for (int off = 32; off >= 1; off >>= 1)
-    v += wave::fetch(v, (int)(wave::lane() ^ (uint32_t)off));
-  return v;
-}
-
-// Per-lane counts of one thread, summed over the workgroup (wave sums, then
-// LDS) and added with one global atomic per nonzero counter per workgroup.
-// The counters share one cache line, where device atomics serialise: one
-// atomic per counter per wave took 0.28 ms of a 0.63 ms launch of 2^20
-// threads.  Every thread of the workgroup calls it once, at the end.
-struct lane_counts {
-  uint32_t pairs = 0, misses = 0, bad = 0, mallocs = 0;
-  uint64_t bytes = 0;
-};
-
-__device__ inline void add_counters(uint64_t* ctr, const lane_counts& c) {
-  __shared__ unsigned long long wg[5];
-  const uint64_t v[5] = {wave_sum(c.pairs), wave_sum(c.misses), wave_sum(c.bad),
-                         wave_sum(c.bytes), wave_sum(c.mallocs)};
-  if (threadIdx.x < 5) wg[threadIdx.x] = 0;
-  __syncthreads();
-  if (wave::lane() == 0) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-      if (v[i]) atomicAdd(&wg[i], (unsigned long long)v[i]);
-  }
-  __syncthreads();
-  if (threadIdx.x < 5 && wg[threadIdx.x])
-    utils::atomic_add(&ctr[threadIdx.x], (uint64_t)wg[threadIdx.x]);
-}
+#include "lane_counts.hpp"
 
 // Payload fill and verify, spread over the whole wave: the buffers of the
 // participating lanes are cut into 16-byte chunks numbered buffer after
@@ -439,15 +408,6 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < kCopyVecs; ++k)
       if (base + (uint64_t)k * 64 + lane < total) __builtin_nontemporal_store(v[k], &dst[to[k]]);
   }
-}
-
-static unsigned grid_for(uint64_t n, uint32_t threads) {
-  return (unsigned)((n + threads - 1) / threads);
-}
-
-static int launch_status(const char* what) {
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? GAL_OK : hip_fail(e, what);
 }
 
 template <int kThreads>

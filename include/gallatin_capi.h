@@ -245,6 +245,15 @@ int gal_run_churn(gal_allocator *a, uint64_t seed, uint64_t nthreads,
 int gal_run_big(gal_allocator *a, uint64_t seed, uint32_t step, uint64_t n_small,
                 uint64_t n_large, void **ptrs_dev, uint64_t *counters_dev, void *stream);
 
+/* The same churn with one 64-bit word of each allocation written (its
+ * (thread, round) pattern) and checked before free, as the reference's
+ * churn test writes one word (gallatin_churn.cu:37-95): batched 0 runs the
+ * per-thread path (`threads` per block), 1 the workgroup-batched path of
+ * gal_run_churn_batched (`workgroups`, `threads` as there). */
+int gal_run_churn_word(gal_allocator *a, uint64_t seed, uint64_t nthreads,
+                       uint32_t rounds, uint64_t *counters_dev, uint32_t batched,
+                       uint32_t workgroups, uint32_t threads, void *stream);
+
 /* The same churn through the workgroup-batched path (batched.hpp): one
  * persistent launch of `workgroups` workgroups (0 = fill the device) of
  * `threads` threads (256, 512 or 1024; 0 = 1024) walks chunks of `threads`

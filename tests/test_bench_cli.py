@@ -58,6 +58,11 @@ def _stub_result():
                          "traffic_frac_of_copy": 0.72, "traffic_frac_of_scatter_2048": 0.88},
         "tree_search": {"workload": "t" * 100, "searches_per_s": 3e9},
         "plain_small": dict(sec, coherent_payload=dict(sec), partitioned_lazy_boot=dict(sec)),
+        "churn_one_word": {"workload": "w" * 150, "value": 1.3e10,
+                           "batched_over_plain": 1.6,
+                           "batched": dict(sec, kernel="k" * 30),
+                           "plain": dict(sec, kernel="k" * 12),
+                           "roofline": {"atomic_frac": 0.99}},
         "block_multiseg": dict(sec, pairs_per_step=1300.0, misses_per_step=2800.0,
                                fresh_after_drain=True, steps=20),
         "batched_zipf": dict(sec, best=best, plain_best=2.6e9, batched_over_plain=1.1,
@@ -104,7 +109,7 @@ def test_headline_line_is_compact_and_last(capsys):
         assert k in head["roofline"], k
     for k in ("value", "unit", "cores", "kind", "sample"):
         assert k in head["cpu_baseline"], k
-    assert set(head["workloads"]) == {"plain_small", "block_multiseg", "batched_zipf",
-                                      "serving"}
+    assert set(head["workloads"]) == {"plain_small", "churn_one_word", "block_multiseg",
+                                      "batched_zipf", "serving"}
     # every earlier line is a section of its own, also valid JSON
     assert all("section" in json.loads(x) for x in lines[:-1])

@@ -470,6 +470,33 @@ def test_churn_batched_patterns_and_counts(gpu, partitions, threads):
     a.free_on_device()
 
 
+@pytest.mark.parametrize("batched,threads,partitions", [
+    (False, 256, 0), (True, 1024, 0), (True, 256, 0), (False, 256, 1), (True, 1024, 1)])
+def test_churn_one_word(gpu, batched, threads, partitions):
+    """The churn stream with one word per allocation written and checked (the
+    reference's churn test pattern), per thread and through workgroup
+    batches, with one partition per XCD and with one partition (a word then
+    lives in any XCD's L2 between its write and its check): every pair
+    completes, every word reads back, 16 bytes per pair are counted, and
+    the pool drains to fresh."""
+    import torch
+    n, rounds, seed = 1 << 20, 10, 11
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(
+        8 << 30, 8, print_info=False, slots_min=32, partitions=partitions)
+    assert (a.partitions > 1) == (partitions == 0)
+    ctr = a.run_churn_word(seed, n, rounds, batched=batched, threads=threads)
+    torch.cuda.synchronize()
+    c = ctr.cpu().tolist()
+    pairs = H.expected_churn_pairs(seed, n, rounds)
+    assert c[gpu.CTR_BAD] == 0
+    assert c[gpu.CTR_MISSES] == 0
+    assert c[gpu.CTR_PAIRS] == pairs
+    assert c[gpu.CTR_BYTES] == 16 * pairs
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_global_allocator(gpu):
     import torch
     gpu.init_global_allocator(1 << 30, 42, print_info=False)
