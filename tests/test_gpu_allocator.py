@@ -497,6 +497,37 @@ def test_churn_one_word(gpu, batched, threads, partitions):
     a.free_on_device()
 
 
+def test_churn_one_word_ragged_and_refused(gpu):
+    """No thread, one thread and thread counts that leave the last block or
+    workgroup chunk partly empty, per thread and at every batched
+    workgroup size, one after another on one instance: the counts match
+    the stream, and the pool drains to fresh.  Arguments outside the
+    entry point's range are refused before any launch."""
+    import torch
+    a = gpu.Gallatin(S, 16, 4096).generate_on_device(1 << 30, 5, print_info=False)
+    rounds = 4
+    for k, n in enumerate((0, 1, 1000, 70001)):
+        for batched, threads in ((False, 256), (False, 64), (True, 256), (True, 512),
+                                 (True, 1024)):
+            seed = 100 * k + threads + int(batched)
+            ctr = a.run_churn_word(seed, n, rounds, batched=batched, threads=threads)
+            torch.cuda.synchronize()
+            c = ctr.cpu().tolist()
+            pairs = H.expected_churn_pairs(seed, n, rounds)
+            case = (n, batched, threads)
+            assert c[gpu.CTR_BAD] == 0, case
+            assert c[gpu.CTR_MISSES] == 0, case
+            assert c[gpu.CTR_PAIRS] == pairs, case
+            assert c[gpu.CTR_BYTES] == 16 * pairs, case
+    for kw in ({"batched": True, "threads": 300}, {"batched": True, "threads": 128}):
+        with pytest.raises(gpu.GallatinError):
+            a.run_churn_word(1, 1000, rounds, **kw)
+    assert a.info()["errors"] == 0
+    a.drain()
+    check_fresh(a)
+    a.free_on_device()
+
+
 def test_global_allocator(gpu):
     import torch
     gpu.init_global_allocator(1 << 30, 42, print_info=False)
