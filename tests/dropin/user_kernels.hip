@@ -321,3 +321,72 @@ extern "C" int user_structures(uint64_t n, unsigned long long* counts_out) {
   (void)hipFree(seen);
   return e == hipSuccess ? 0 : -13;
 }
+
+namespace {
+
+// The reference's churn test kernel (tests/src/gallatin_churn.cu:37-95) as
+// a user writes it against these headers: every round a thread hashes its
+// running value into a size, mallocs it (retrying up to ten times),
+// exchanges its id into the allocation's first word (which must hold 0:
+// the pool starts zeroed and every user puts 0 back), exchanges 0 back
+// (which must return its id) and frees it.  The reference prints each
+// double malloc; here they are counted.
+template <typename allocator>
+__global__ void pointer_churn(allocator* gallatin, uint64_t num_allocs, int num_rounds,
+                              uint64_t min_size, uint64_t max_size,
+                              unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= num_allocs) return;
+  uint64_t hash = tid;
+  for (int i = 0; i < num_rounds; ++i) {
+    hash = gallatin::hashers::MurmurHash64A(&hash, sizeof(uint64_t), i);
+    uint64_t size = hash % max_size;
+    if (size < min_size) size = min_size;
+    __threadfence();
+    void* a = gallatin->malloc(size);
+    for (int retry = 0; a == nullptr && retry < 10; ++retry) {
+      __threadfence();
+      a = gallatin->malloc(size);
+    }
+    if (a == nullptr) {
+      atomicAdd(&counts[0], 1ULL);
+      continue;
+    }
+    auto* word = static_cast<unsigned long long*>(a);
+    if (atomicExch(word, (unsigned long long)tid) != 0ULL) atomicAdd(&counts[1], 1ULL);
+    if (atomicExch(word, 0ULL) != (unsigned long long)tid) atomicAdd(&counts[2], 1ULL);
+    gallatin->free(a);
+  }
+}
+
+}  // namespace
+
+// tests/src/gallatin_churn.cu:98-165 gallatin_pointer_churn: an instance of
+// num_bytes (seed 111), one launch of num_allocs threads in blocks of 512,
+// then print_info.  counts_out: [0] misses, [1] double mallocs seen at the
+// first exchange, [2] seen at the second, [3] allocator errors after the
+// launch.
+extern "C" int user_churn(uint64_t num_bytes, uint64_t num_allocs, int num_rounds,
+                          uint64_t min_size, uint64_t max_size,
+                          unsigned long long* counts_out) {
+  unsigned long long* counts = nullptr;
+  if (num_allocs == 0 || max_size == 0) return -1;
+  if (hipMalloc(&counts, 3 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMemset(counts, 0, 3 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  my_allocator* alloc = my_allocator::generate_on_device(num_bytes, 111, false);
+  if (alloc == nullptr) return -3;
+  const unsigned blocks = (unsigned)((num_allocs - 1) / 512 + 1);
+  pointer_churn<my_allocator><<<blocks, 512>>>(alloc, num_allocs, num_rounds, min_size,
+                                                max_size, counts);
+  if (hipDeviceSynchronize() != hipSuccess) return -2;
+  alloc->print_info();
+  gal_info info;
+  const int irc = gal_get_info(gal_from_device_state(alloc), &info);
+  my_allocator::free_on_device(alloc);
+  if (irc != 0) return -4;
+  const hipError_t e = hipMemcpy(counts_out, counts, 3 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost);
+  counts_out[3] = info.errors;
+  (void)hipFree(counts);
+  return e == hipSuccess ? 0 : -2;
+}

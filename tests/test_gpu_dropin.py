@@ -24,7 +24,9 @@ def _lib():
 
 
 def test_user_library_loads_and_exports():
-    assert hasattr(_lib(), "user_run") and hasattr(_lib(), "user_structures")
+    L = _lib()
+    for name in ("user_run", "user_structures", "user_table_accessors", "user_churn"):
+        assert hasattr(L, name), name
 
 
 @pytest.mark.gpu
@@ -69,3 +71,27 @@ def test_user_kernels_table_accessors(size):
     counts = (ctypes.c_ulonglong * 4)()
     assert L.user_table_accessors(1 << 30, 1 << 16, size, counts) == 0
     assert list(counts) == [0, 0, 0, 0], "misses, accessor, id, block-count errors"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("num_bytes,threads,rounds,min_size,max_size", [
+    (1000 * (16 << 20), 1 << 20, 10, 16, 4096),   # the test's own defaults
+    (4 << 30, 1 << 18, 10, 16, 4096),             # one partition
+    (8 << 30, 1 << 16, 4, 16, 65536),             # runs of largest slices too
+])
+def test_user_churn_kernel(num_bytes, threads, rounds, min_size, max_size):
+    """The reference's churn test kernel (tests/src/gallatin_churn.cu:37-165)
+    written against the headers in the user unit: each thread mallocs a
+    hashed size every round, exchanges its id into the first word, which
+    must hold 0, exchanges 0 back, which must return its id, and frees.
+    Every allocation is freed in its round, so the pools leave ample room:
+    no miss, no double malloc, no allocator error."""
+    import torch
+    torch.cuda.set_device(0)
+    L = _lib()
+    L.user_churn.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                             ctypes.c_uint64, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.user_churn.restype = ctypes.c_int
+    counts = (ctypes.c_ulonglong * 4)()
+    assert L.user_churn(num_bytes, threads, rounds, min_size, max_size, counts) == 0
+    assert list(counts) == [0, 0, 0, 0], "misses, double mallocs (claim, return), errors"
