@@ -390,3 +390,76 @@ extern "C" int user_churn(uint64_t num_bytes, uint64_t num_allocs, int num_round
   (void)hipFree(counts);
   return e == hipSuccess ? 0 : -2;
 }
+
+namespace {
+
+// tests/src/gallatin_single_thread.cu:38-86: one thread mallocs num_allocs
+// allocations of one size in a row through the global allocator, writing
+// each index into its first word, and a second launch frees them in order,
+// checking the index first.  Counted here instead of printed.
+__global__ void single_thread_malloc(uint64_t num_allocs, uint64_t size, uint64_t** ptrs,
+                                     unsigned long long* counts) {
+  if (gallatin::utils::get_tid() != 0) return;
+  for (uint64_t i = 0; i < num_allocs; ++i) {
+    uint64_t* p = static_cast<uint64_t*>(global_malloc(size));
+    ptrs[i] = p;
+    if (p == nullptr) {
+      counts[0] += 1;
+      continue;
+    }
+    p[0] = i;
+    __threadfence();
+  }
+}
+
+__global__ void single_thread_free(uint64_t num_allocs, uint64_t** ptrs,
+                                   unsigned long long* counts) {
+  if (gallatin::utils::get_tid() != 0) return;
+  for (uint64_t i = 0; i < num_allocs; ++i) {
+    uint64_t* p = ptrs[i];
+    if (p == nullptr) continue;
+    if (p[0] != i) counts[1] += 1;
+    global_free(p);
+    __threadfence();
+  }
+}
+
+}  // namespace
+
+// gallatin_single_thread.cu:91-140: a global allocator of 1000 segments,
+// then num_rounds rounds of one single-thread malloc launch and one
+// single-thread free launch.  counts_out: [0] misses, [1] indices that did
+// not read back, [2] allocator errors, [3] segments not free after the
+// rounds and a drain.
+extern "C" int user_single_thread(uint64_t num_allocs, int num_rounds, uint64_t size,
+                                  unsigned long long* counts_out) {
+  if (num_allocs == 0) return -1;
+  init_global_allocator(1000ULL * (16ULL << 20), 42, false);
+  uint64_t** ptrs = nullptr;
+  unsigned long long* counts = nullptr;
+  if (hipMalloc(&ptrs, num_allocs * sizeof(uint64_t*)) != hipSuccess) return -1;
+  if (hipMalloc(&counts, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMemset(counts, 0, 2 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  for (int r = 0; r < num_rounds; ++r) {
+    single_thread_malloc<<<1, 1>>>(num_allocs, size, ptrs, counts);
+    single_thread_free<<<1, 1>>>(num_allocs, ptrs, counts);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+  }
+  print_global_stats();
+  global_allocator_type* g = nullptr;
+  if (hipMemcpyFromSymbol(&g, HIP_SYMBOL(global_gallatin), sizeof(g)) != hipSuccess) return -2;
+  gal_allocator* h = g ? gal_from_device_state(g) : nullptr;
+  gal_info info;
+  int rc = h ? gal_drain(h, nullptr) : -4;
+  if (rc == 0 && hipDeviceSynchronize() != hipSuccess) rc = -2;
+  if (rc == 0) rc = gal_get_info(h, &info);
+  free_global_allocator();
+  const hipError_t e = hipMemcpy(counts_out, counts, 2 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost);
+  (void)hipFree(ptrs);
+  (void)hipFree(counts);
+  if (rc != 0) return -4;
+  counts_out[2] = info.errors;
+  counts_out[3] = info.num_segments - info.segments_free;
+  return e == hipSuccess ? 0 : -2;
+}

@@ -25,7 +25,8 @@ def _lib():
 
 def test_user_library_loads_and_exports():
     L = _lib()
-    for name in ("user_run", "user_structures", "user_table_accessors", "user_churn"):
+    for name in ("user_run", "user_structures", "user_table_accessors", "user_churn",
+                 "user_single_thread"):
         assert hasattr(L, name), name
 
 
@@ -95,3 +96,24 @@ def test_user_churn_kernel(num_bytes, threads, rounds, min_size, max_size):
     counts = (ctypes.c_ulonglong * 4)()
     assert L.user_churn(num_bytes, threads, rounds, min_size, max_size, counts) == 0
     assert list(counts) == [0, 0, 0, 0], "misses, double mallocs (claim, return), errors"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size,n", [(16, 10000), (4096, 10000), (1 << 20, 1000),
+                                    (32 << 20, 100)])
+def test_user_single_thread(size, n):
+    """The reference's single-thread test (tests/src/gallatin_single_thread.cu:
+    38-140) in the user unit: one thread mallocs n allocations of one size in
+    a row and then frees them, two rounds, through the global allocator.
+    Slices (16 B, 4 KiB: several blocks, 4 KiB ones several segments), whole
+    blocks (1 MiB) and two-segment runs (32 MiB): every index reads back, no
+    miss, no error, and after a drain every segment is free."""
+    import torch
+    torch.cuda.set_device(0)
+    L = _lib()
+    L.user_single_thread.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_ulonglong)]
+    L.user_single_thread.restype = ctypes.c_int
+    counts = (ctypes.c_ulonglong * 4)()
+    assert L.user_single_thread(n, 2, size, counts) == 0
+    assert list(counts) == [0, 0, 0, 0], "misses, bad indices, errors, segments not free"
