@@ -463,3 +463,97 @@ extern "C" int user_single_thread(uint64_t num_allocs, int num_rounds, uint64_t 
   counts_out[3] = info.num_segments - info.segments_free;
   return e == hipSuccess ? 0 : -2;
 }
+
+namespace {
+
+// tests/src/gallatin_host_test.cu:40-185 on a pool in host memory: the
+// allocation kernels of that test on an instance the user owns, and its
+// overlap check, which finds each allocation's block of the one size class
+// from the pool base (alloc->table->memory) and counts allocations per
+// block.
+__global__ void host_alloc(my_allocator* alloc, uint64_t n, uint64_t size, uint64_t** ptrs,
+                           unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t* p = static_cast<uint64_t*>(alloc->malloc(size));
+  ptrs[tid] = p;
+  if (p == nullptr) {
+    atomicAdd(&counts[0], 1ULL);
+    return;
+  }
+  p[0] = tid;
+  __threadfence();
+}
+
+__global__ void host_overlaps(my_allocator* alloc, uint64_t slice_size, uint64_t** ptrs,
+                              uint64_t n, unsigned long long* per_block, uint64_t nblocks,
+                              unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n || ptrs[tid] == nullptr) return;
+  const char* start_of_memory = alloc->table->memory;
+  const uint64_t offset = ((uint64_t)ptrs[tid] - (uint64_t)start_of_memory) / slice_size;
+  if (offset / 4096 < nblocks)
+    atomicAdd(&per_block[offset / 4096], 1ULL);
+  else
+    atomicAdd(&counts[2], 1ULL);  // outside the pool
+}
+
+__global__ void host_block_counts(const unsigned long long* per_block, uint64_t nblocks,
+                                  unsigned long long* counts) {
+  const uint64_t b = gallatin::utils::get_tid();
+  if (b >= nblocks) return;
+  if (per_block[b] > 4096) atomicAdd(&counts[2], 1ULL);
+  if (per_block[b]) atomicAdd(&counts[3], per_block[b]);
+}
+
+__global__ void host_free(my_allocator* alloc, uint64_t n, uint64_t** ptrs,
+                          unsigned long long* counts) {
+  const uint64_t tid = gallatin::utils::get_tid();
+  if (tid >= n) return;
+  uint64_t* p = ptrs[tid];
+  if (p == nullptr) return;
+  if (p[0] != tid) atomicAdd(&counts[1], 1ULL);
+  alloc->free(p);
+  __threadfence();
+}
+
+}  // namespace
+
+// counts_out: [0] misses, [1] ids that did not read back, [2] blocks
+// holding more than 4096 allocations and allocations outside the pool,
+// [3] allocations counted in blocks
+// (n - misses when none is lost), [4] allocator errors.  size must be a
+// slice class (a power of two in [16, 4096]).
+extern "C" int user_host_pool(uint64_t pool_bytes, uint64_t n, uint64_t size,
+                              unsigned long long* counts_out) {
+  if (n == 0 || size < 16 || size > 4096 || (size & (size - 1))) return -1;
+  my_allocator* alloc = my_allocator::generate_on_device_host(pool_bytes, 42, false);
+  if (alloc == nullptr) return -3;
+  gal_info info;
+  if (gal_get_info(gal_from_device_state(alloc), &info) != 0) return -4;
+  const uint64_t nblocks = info.num_segments * (info.bytes_per_segment / (size * 4096));
+  uint64_t** ptrs = nullptr;
+  unsigned long long *counts = nullptr, *per_block = nullptr;
+  if (hipMalloc(&ptrs, n * sizeof(uint64_t*)) != hipSuccess) return -1;
+  if (hipMalloc(&counts, 4 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMalloc(&per_block, nblocks * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMemset(counts, 0, 4 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMemset(per_block, 0, nblocks * sizeof(unsigned long long)) != hipSuccess) return -1;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  host_alloc<<<blocks, 256>>>(alloc, n, size, ptrs, counts);
+  host_overlaps<<<blocks, 256>>>(alloc, size, ptrs, n, per_block, nblocks, counts);
+  host_block_counts<<<(unsigned)((nblocks + 255) / 256), 256>>>(per_block, nblocks, counts);
+  host_free<<<blocks, 256>>>(alloc, n, ptrs, counts);
+  if (hipDeviceSynchronize() != hipSuccess) return -2;
+  alloc->print_info();
+  const int irc = gal_get_info(gal_from_device_state(alloc), &info);
+  my_allocator::free_on_device(alloc);
+  const hipError_t e = hipMemcpy(counts_out, counts, 4 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost);
+  (void)hipFree(ptrs);
+  (void)hipFree(counts);
+  (void)hipFree(per_block);
+  if (irc != 0) return -4;
+  counts_out[4] = info.errors;
+  return e == hipSuccess ? 0 : -2;
+}

@@ -26,7 +26,7 @@ def _lib():
 def test_user_library_loads_and_exports():
     L = _lib()
     for name in ("user_run", "user_structures", "user_table_accessors", "user_churn",
-                 "user_single_thread"):
+                 "user_single_thread", "user_host_pool"):
         assert hasattr(L, name), name
 
 
@@ -117,3 +117,23 @@ def test_user_single_thread(size, n):
     counts = (ctypes.c_ulonglong * 4)()
     assert L.user_single_thread(n, 2, size, counts) == 0
     assert list(counts) == [0, 0, 0, 0], "misses, bad indices, errors, segments not free"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [64, 4096])
+def test_user_host_pool_overlaps(size):
+    """The reference's host-pool test (tests/src/gallatin_host_test.cu:40-260)
+    in the user unit: an instance from generate_on_device_host, 2^18
+    allocations of one slice class written with their ids, and that test's
+    overlap check, which counts allocations per block from
+    alloc->table->memory: nothing missed, every id reads back, no block
+    holds more than 4096 and the blocks hold every allocation."""
+    import torch
+    torch.cuda.set_device(0)
+    L = _lib()
+    L.user_host_pool.argtypes = [ctypes.c_uint64] * 3 + [ctypes.POINTER(ctypes.c_ulonglong)]
+    L.user_host_pool.restype = ctypes.c_int
+    n = 1 << 18
+    counts = (ctypes.c_ulonglong * 5)()
+    assert L.user_host_pool(4 << 30, n, size, counts) == 0
+    assert list(counts) == [0, 0, 0, n, 0], "misses, ids, blocks, counted, errors"
