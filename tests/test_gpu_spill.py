@@ -15,7 +15,10 @@ global_allocator.cuh:100-357).
 
 The reference's tests for these (global_exceed_test.cu:117-150,
 poison_test.cu:97-194) print and always pass, so these invariants are
-what pins the behaviour.
+what pins the behaviour.  Its host-instance tests (global_host_test.cu and
+global_combined_test.cu, 34-209: one-size mallocs through the global host
+instance, freed through it) go through the same host instance that the
+combined and fused tests here place allocations in and free them from.
 """
 import ctypes
 
